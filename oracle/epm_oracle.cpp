@@ -1,0 +1,694 @@
+/*
+ * epm_oracle.cpp — CPU restatement of the reference's per-k band solve.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * smoke() entry and bench.py's cpu_baseline leg load it.  The product path
+ * (libuepm.so) never links, calls or falls back to it.
+ *
+ * What it restates (reference file:line):
+ *   basis          BandStructure::GenerateBasisVectors  src/EPP/BandStructure.cpp:26-57
+ *   k path         SymmetryPoints::GeneratePoints       src/EPP/SymmetryPoints.cpp:21-147
+ *   local V        Pseudopotential::GetValue            src/EPP/Pseudopotential.cpp:24-56
+ *   H(k)           Hamiltonian::SetMatrix               src/EPP/Hamiltonian.cpp:50-123
+ *   nonlocal       Material::compute_pseudopotential_non_local_correction,
+ *                  F_l_function, F_2_function_gaussian  src/EPP/Material.cpp:87-208
+ *   Bessel         generalized_bessel, J_neg_v, i_2     src/EPP/bessel_func.hpp:17-52
+ *   spin-orbit     SpinOrbitCorrection                  src/EPP/SpinOrbitFunctional.cpp:20-138
+ *   hermitisation  Hamiltonian::Diagonalize             src/EPP/Hamiltonian.cpp:125-134
+ *   gradient       compute_gradient_at_level            src/EPP/Hamiltonian.cpp:146-194
+ *   k loop         BandStructure::Compute_parallel      src/EPP/BandStructure.cpp:151-201
+ *   BZ test        is_in_first_BZ                       src/EPP/DielectricFunction.cpp:35-40
+ *   VBM shift      AdjustValues / FindBandGap           src/EPP/BandStructure.cpp:203-262
+ *
+ * Every floating-point expression keeps the reference's operation order and
+ * uses the same libstdc++ special functions (std::cyl_bessel_j,
+ * std::cyl_neumann).  Build without -ffast-math and with -ffp-contract=off.
+ *
+ * The eigensolver stands in for Eigen 5.0.0's SelfAdjointEigenSolver (absent
+ * from this image): Householder reduction to real tridiagonal form followed by
+ * the implicit-shift QL iteration (Wilkinson-Reinsch tql2, Handbook for
+ * Automatic Computation vol. II, 1971), the same backward-stable algorithm
+ * family; eigenvalues agree with Eigen to ~c*N*eps*||H||.
+ */
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <omp.h>
+
+#include "../include/uepm.h"
+
+namespace {
+
+using cplx = std::complex<double>;
+
+/* src/Common/physical_constants.hpp:18-41 */
+constexpr double kPi          = 3.14159265358979323846;  // M_PI == std::numbers::pi
+constexpr double kH           = 6.62607015e-34;
+constexpr double kQe          = 1.602176634e-19;
+constexpr double kHbar        = kH / (2.0 * kPi);
+constexpr double kMe          = 9.1093837015e-31;
+constexpr double kBohr        = 5.29177210903e-11;
+constexpr double kHartreeJ    = 4.3597447222071e-18;
+constexpr double kHartreeEV   = kHartreeJ / kQe;
+constexpr double kRydEV       = kHartreeEV / 2.0;
+
+struct V3 {
+    double x, y, z;
+};
+inline double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline double length(const V3& a) { return std::sqrt(dot(a, a)); }  // Vector3D.inl:116-121
+inline V3 scale(const V3& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 sub(const V3& a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+/* bessel_func.hpp:17-45 */
+double J_neg_v(double v, double x) {
+    return std::cos(-v * kPi) * std::cyl_bessel_j(-v, x) - std::sin(-v * kPi) * std::cyl_neumann(-v, x);
+}
+double gen_bessel_cyl(double nu, double x) { return nu >= 0 ? std::cyl_bessel_j(nu, x) : J_neg_v(nu, x); }
+double gen_bessel(double nu, double x) {
+    constexpr double epsilon = 1e-14;
+    if (x < 0) throw std::invalid_argument("Bessel function x argument must be positive.");
+    if (x < epsilon && nu == 0) return 1.0;
+    if (x < epsilon && nu == 1) return 0.0;
+    if (x < epsilon && nu >= 2) return 0.0;
+    return std::sqrt(kPi / (2.0 * x)) * gen_bessel_cyl(nu + 0.5, x);
+}
+/* bessel_func.hpp:47-52 */
+double i2(double x) {
+    if (x == 0.0) return 0.0;
+    return ((x * x + 3) * std::sinh(x) - 3 * x * std::cosh(x)) / (x * x * x);
+}
+
+/* Material.cpp:87-109 */
+double F_l(const V3& K1, const V3& K2, double R, int l) {
+    constexpr double EPSILON = 1.0e-4;
+    const double n1 = length(K1), n2 = length(K2);
+    if (std::fabs(n1 - n2) > EPSILON) {
+        const double pre = std::pow(R, 2.0) / (n1 * n1 - n2 * n2);
+        const double F = n1 * gen_bessel(l + 1, n1 * R) * gen_bessel(l, n2 * R) -
+                         n2 * gen_bessel(l + 1, n2 * R) * gen_bessel(l, n1 * R);
+        return pre * F;
+    } else if (n1 > EPSILON) {
+        const double pre = std::pow(R, 3.0) / (2.0);
+        const double F = std::pow(gen_bessel(l, n1 * R), 2.0) - gen_bessel(l - 1, n1 * R) * gen_bessel(l + 1, n1 * R);
+        return pre * F;
+    }
+    return (l == 0) ? std::pow(R, 3.0) / (3.0) : 0.0;
+}
+
+/* Material.cpp:111-116 */
+double F_2_gauss(const V3& K1, const V3& K2, double R) {
+    const double n1 = length(K1), n2 = length(K2);
+    const double arg = 0.5 * (R * R) * n1 * n2;
+    return i2(arg) * std::exp(-0.25 * (n1 * n1 + n2 * n2) * R * R);
+}
+
+inline double lattice_m(const uepm_material& m) { return 1.0e-10 * m.lattice_constant_angstrom; }  // Material.h:108-111
+
+/* Material.cpp:139-208 */
+cplx nl_element(const uepm_material& m, const V3& K1n, const V3& K2n, const V3& tau) {
+    const double diag_factor = std::pow(kHbar, 2) / (2.0 * kMe * kQe);
+    const double a = lattice_m(m);
+    const double fourier = 2.0 * kPi / a;
+    const V3 Gdiff = sub(K1n, K2n);
+    const V3 K1 = scale(K1n, fourier), K2 = scale(K2n, fourier);
+    const double n1 = length(K1), n2 = length(K2);
+    double cos_angle;
+    {  // Vector3D.h:75-79
+        const double dp = dot(K1, K2);
+        const double np = length(K1) * length(K2);
+        cos_angle = (np < 1.0e-13) ? 1.0 : dp / (length(K1) * length(K2));
+    }
+    const double atomic_volume = (1.0 / 8.0) * std::pow(a, 3.0);  // Material.h:130
+    const double V_pre = 4.0 * kPi / atomic_volume;
+    const double leg0 = 1.0;
+    const double leg2 = 0.5 * (3 * cos_angle * cos_angle - 1);
+    const double kF = std::pow(96 * kPi * kPi, 1.0 / 3.0) / a;  // Material.h:170
+
+    double V_anion = 0;
+    const double A0a = m.alpha_0_anion + diag_factor * m.beta_0_anion * (n1 * n2 - std::pow(kF, 2.0));
+    const double F0a = (m.R0_anion == 0.0) ? 0.0 : F_l(K1, K2, m.R0_anion, 0);
+    V_anion += V_pre * A0a * (2 * 0 + 1) * 1.0 * F0a;
+    if (m.A2_anion != 0) {
+        double V2 = 0.0;
+        if (m.nl_well_type == 0) {
+            const double F2 = F_l(K1, K2, m.R2_anion, 2);
+            V2 = V_pre * m.A2_anion * (2 * 2 + 1) * leg2 * F2;
+        } else {
+            const double F2 = F_2_gauss(K1, K2, m.R2_anion);
+            V2 = 5.0 * std::pow(kPi, 1.5) * (std::pow(m.R2_anion, 3.0) / atomic_volume) * m.A2_anion * leg2 * F2;
+        }
+        V_anion += V2;
+    }
+    double V_cation = 0;
+    const double F0c = (m.R0_cation == 0) ? 0.0 : F_l(K1, K2, m.R0_cation, 0);
+    const double A0c = m.alpha_0_cation + m.beta_0_cation * diag_factor * (n1 * n2 - std::pow(kF, 2.0));
+    V_cation += V_pre * A0c * (2 * 0 + 1) * leg0 * F0c;
+    if (m.A2_cation != 0.0) {
+        double V2 = 0.0;
+        if (m.nl_well_type == 0) {
+            const double F2 = F_l(K1, K2, m.R2_cation, 2);
+            V2 = V_pre * m.A2_cation * (2 * 2 + 1) * leg2 * F2;
+        } else {
+            const double F2 = F_2_gauss(K1, K2, m.R2_cation);
+            V2 = 5.0 * std::pow(kPi, 1.5) * (std::pow(m.R2_cation, 3.0) / atomic_volume) * m.A2_cation * leg2 * F2;
+        }
+        V_cation += V2;
+    }
+    const double Vs = 1.0 * (V_anion + V_cation) / 2.0;
+    const double Va = 1.0 * (V_anion - V_cation) / 2.0;
+    const V3 tau_over_a = {tau.x / a, tau.y / a, tau.z / a};
+    const double Gtau = dot(tau_over_a, Gdiff);
+    return cplx(std::cos(2.0 * kPi * Gtau) * Vs, std::sin(2.0 * kPi * Gtau) * Va);
+}
+
+/* SpinOrbitFunctional.cpp:20-49 */
+double B_radial(int period, double Klen, double xi) {
+    const double kappa = Klen * kBohr / xi;
+    switch (period) {
+        case 2: return 1.0 / std::pow(1.0 + kappa * kappa, 3.0);
+        case 3: return (5.0 - kappa * kappa) / (5.0 * std::pow(1.0 + kappa * kappa, 4.0));
+        case 4: return (5.0 - 3.0 * kappa * kappa) / (5.0 * std::pow(1.0 + kappa * kappa, 5.0));
+        default: throw std::runtime_error("SpinOrbitCorrection: Unsupported period.");
+    }
+}
+
+/* GCC's __muldc3 finite path: (ac - bd) + i(ad + bc), products rounded separately */
+inline cplx cmul(cplx p, cplx q) {
+    const double ac = p.real() * q.real(), bd = p.imag() * q.imag();
+    const double ad = p.real() * q.imag(), bc = p.imag() * q.real();
+    return cplx(ac - bd, ad + bc);
+}
+
+/* SpinOrbitFunctional.cpp:100-138; out = [s00, s01, s10, s11] */
+void soc_block(const uepm_material& m, const V3& K, const V3& Kp, const V3& G, const V3& Gp, const V3& tau, cplx out[4]) {
+    const double a = lattice_m(m);
+    const double kfac = (2.0 * kPi) / a;
+    const V3 Kphys = scale(K, kfac), Kpphys = scale(Kp, kfac);
+    const double lK = length(Kphys), lKp = length(Kpphys);
+    const double l1 = m.mu_soc * B_radial(m.period_cation, lK, m.radial_extent_cation) *
+                      B_radial(m.period_cation, lKp, m.radial_extent_cation);
+    const double l2 = m.alpha_soc * m.mu_soc * B_radial(m.period_anion, lK, m.radial_extent_anion) *
+                      B_radial(m.period_anion, lKp, m.radial_extent_anion);
+    const double lsym = (l1 + l2) / 2.0;
+    const double lanti = (l1 - l2) / 2.0;
+    const double a00 = K.x * Kp.y - K.y * Kp.x;
+    const double cxr = K.y * Kp.z - K.z * Kp.y;
+    const double cyi = K.x * Kp.z - K.z * Kp.x;
+    const cplx p00(a00, 0.0), p01(cxr, cyi), p10(cxr, -cyi), p11(-(K.x * Kp.y - K.y * Kp.x), 0.0);
+    const V3 dG = sub(G, Gp);
+    const double Gtau = kfac * dot(tau, dG);
+    const cplx phase(lanti * std::sin(Gtau), -lsym * std::cos(Gtau));
+    out[0] = cmul(p00, phase);
+    out[1] = cmul(p01, phase);
+    out[2] = cmul(p10, phase);
+    out[3] = cmul(p11, phase);
+}
+
+/* Pseudopotential.cpp:24-56 */
+cplx local_V(const uepm_material& m, int gx, int gy, int gz, const V3& tau, double a) {
+    const int G2 = gx * gx + gy * gy + gz * gz;
+    const double kfac = 2.0 * kPi / a;
+    const double Gtau = kfac * (tau.x * gx + tau.y * gy + tau.z * gz);
+    double VS = 0.0, VA = 0.0;
+    switch (G2) {
+        case 3: VS = m.V_S[0]; VA = m.V_A[0]; break;
+        case 4: VS = m.V_S[1]; VA = m.V_A[1]; break;
+        case 8: VS = m.V_S[2]; VA = m.V_A[2]; break;
+        case 11: VS = m.V_S[3]; VA = m.V_A[3]; break;
+        default: break;
+    }
+    return cplx(VS * std::cos(Gtau), VA * std::sin(Gtau));
+}
+
+/* Hamiltonian.cpp:50-123 (before hermitisation).  H: dim x dim column-major. */
+void assemble(const uepm_material& m, const int32_t* G, int N, const double* k, bool nl, bool soc, cplx* H) {
+    const double a = lattice_m(m);
+    constexpr double one_eight = 1.0 / 8.0;
+    const V3 tau = {one_eight * a, one_eight * a, one_eight * a};
+    const double tpa = 2.0 * kPi / a;
+    const int dim = soc ? 2 * N : N;
+    std::vector<cplx> M(static_cast<size_t>(N) * N);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i)
+            M[i + (size_t)j * N] = local_V(m, G[3 * i] - G[3 * j], G[3 * i + 1] - G[3 * j + 1], G[3 * i + 2] - G[3 * j + 2], tau, a);
+    std::vector<V3> kpg(N);
+    for (int i = 0; i < N; ++i) kpg[i] = {k[0] + G[3 * i], k[1] + G[3 * i + 1], k[2] + G[3 * i + 2]};
+    const double diag_factor = (kHbar * kHbar) / (2.0 * kMe * kQe);
+    for (int i = 0; i < N; ++i) {
+        const double KG2 = tpa * tpa * diag_factor * dot(kpg[i], kpg[i]);
+        M[i + (size_t)i * N] += cplx(KG2, 0.0);
+    }
+    if (nl) {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) M[i + (size_t)j * N] += nl_element(m, kpg[i], kpg[j], tau);
+    }
+    if (!soc) {
+        std::copy(M.begin(), M.end(), H);
+        return;
+    }
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const V3 Gi = {(double)G[3 * i], (double)G[3 * i + 1], (double)G[3 * i + 2]};
+            const V3 Gj = {(double)G[3 * j], (double)G[3 * j + 1], (double)G[3 * j + 2]};
+            cplx s[4];
+            soc_block(m, kpg[i], kpg[j], Gi, Gj, tau, s);
+            const cplx base = M[i + (size_t)j * N];
+            H[i + (size_t)j * dim] = base + s[0];
+            H[i + (size_t)(j + N) * dim] = s[1];
+            H[(i + N) + (size_t)j * dim] = s[2];
+            H[(i + N) + (size_t)(j + N) * dim] = base + s[3];
+        }
+}
+
+/* Hamiltonian.cpp:127: matrix = 0.5 * (matrix + matrix.adjoint()) */
+void hermitize(int n, cplx* H) {
+    std::vector<cplx> T(H, H + (size_t)n * n);
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            const cplx s = T[i + (size_t)j * n] + std::conj(T[j + (size_t)i * n]);
+            H[i + (size_t)j * n] = cplx(0.5 * s.real(), 0.5 * s.imag());
+        }
+}
+
+/* Householder reduction of a Hermitian matrix (lower triangle) to real
+ * symmetric tridiagonal T = Q^H A Q, Q = H_0 H_1 ... H_{n-2},
+ * H_j = I - tau_j v_j v_j^H, v_j(0) = 1 stored below the subdiagonal. */
+void tridiagonalize(int n, cplx* A, double* d, double* e, cplx* tau) {
+    std::vector<cplx> p(n);
+    for (int j = 0; j + 1 < n; ++j) {
+        const int m = n - j - 1;
+        cplx* x = A + (j + 1) + (size_t)j * n;  // column j below the diagonal, length m
+        const cplx alpha = x[0];
+        double xn2 = 0.0;
+        for (int i = 1; i < m; ++i) xn2 += std::norm(x[i]);
+        cplx tj(0.0, 0.0);
+        double beta = alpha.real();
+        if (!(xn2 == 0.0 && alpha.imag() == 0.0)) {
+            const double nrm = std::sqrt(std::norm(alpha) + xn2);
+            beta = alpha.real() >= 0 ? -nrm : nrm;
+            tj = cplx((beta - alpha.real()) / beta, -alpha.imag() / beta);
+            const cplx sc = 1.0 / (alpha - beta);
+            for (int i = 1; i < m; ++i) x[i] *= sc;
+        }
+        x[0] = 1.0;
+        e[j] = beta;
+        tau[j] = tj;
+        if (tj != cplx(0.0, 0.0)) {
+            cplx* A22 = A + (j + 1) + (size_t)(j + 1) * n;
+            // p = tau * A22 * v   (A22 Hermitian, lower triangle referenced)
+            for (int r = 0; r < m; ++r) p[r] = 0.0;
+            for (int c = 0; c < m; ++c) {
+                const cplx* col = A22 + (size_t)c * n;
+                p[c] += col[c].real() * x[c];
+                for (int r = c + 1; r < m; ++r) {
+                    p[r] += col[r] * x[c];
+                    p[c] += std::conj(col[r]) * x[r];
+                }
+            }
+            for (int r = 0; r < m; ++r) p[r] *= tj;
+            cplx pv(0.0, 0.0);
+            for (int r = 0; r < m; ++r) pv += std::conj(p[r]) * x[r];
+            const cplx al = -0.5 * tj * pv;
+            for (int r = 0; r < m; ++r) p[r] += al * x[r];
+            // A22 -= v p^H + p v^H (lower triangle)
+            for (int c = 0; c < m; ++c) {
+                cplx* col = A22 + (size_t)c * n;
+                for (int r = c; r < m; ++r) col[r] -= x[r] * std::conj(p[c]) + p[r] * std::conj(x[c]);
+                col[c] = cplx(col[c].real(), 0.0);
+            }
+        }
+        d[j] = A[j + (size_t)j * n].real();
+    }
+    d[n - 1] = A[(n - 1) + (size_t)(n - 1) * n].real();
+    if (n >= 1) e[n - 1] = 0.0;
+}
+
+/* Implicit-shift QL iteration on (d, e), e[i] couples i and i+1.
+ * Z (n x n, column-major, real) accumulates the rotations when non-null. */
+bool tql2(int n, double* d, double* e, double* Z) {
+    for (int l = 0; l < n; ++l) {
+        int iter = 0;
+        int m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 60) return false;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                bool underflow = false;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i];
+                    const double b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) {
+                        d[i + 1] -= p;
+                        e[m] = 0.0;
+                        underflow = true;
+                        break;
+                    }
+                    s = f / r;
+                    c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    if (Z) {
+                        double* zi = Z + (size_t)i * n;
+                        double* zi1 = Z + (size_t)(i + 1) * n;
+                        for (int k = 0; k < n; ++k) {
+                            f = zi1[k];
+                            zi1[k] = s * zi[k] + c * f;
+                            zi[k] = c * zi[k] - s * f;
+                        }
+                    }
+                }
+                if (underflow && i >= l) continue;
+                d[l] -= p;
+                e[l] = g;
+                e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    return true;
+}
+
+/* Eigen-decomposition of a Hermitian matrix (full storage, column-major).
+ * evals ascending; evecs (n x n column-major) when non-null. */
+bool eigh(int n, const cplx* Hin, double* evals, cplx* evecs) {
+    std::vector<cplx> A(Hin, Hin + (size_t)n * n);
+    std::vector<double> d(n), e(n);
+    std::vector<cplx> tau(n);
+    tridiagonalize(n, A.data(), d.data(), e.data(), tau.data());
+    std::vector<double> Z;
+    if (evecs) {
+        Z.assign((size_t)n * n, 0.0);
+        for (int i = 0; i < n; ++i) Z[i + (size_t)i * n] = 1.0;
+    }
+    if (!tql2(n, d.data(), e.data(), evecs ? Z.data() : nullptr)) return false;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return d[p] < d[q]; });
+    for (int i = 0; i < n; ++i) evals[i] = d[order[i]];
+    if (!evecs) return true;
+    // X = Q Z: apply H_{n-2}, ..., H_0 to the real eigenvector columns.
+    for (int c = 0; c < n; ++c) {
+        cplx* x = evecs + (size_t)c * n;
+        const double* z = Z.data() + (size_t)order[c] * n;
+        for (int i = 0; i < n; ++i) x[i] = z[i];
+        for (int j = n - 2; j >= 0; --j) {
+            if (tau[j] == cplx(0.0, 0.0)) continue;
+            const cplx* v = A.data() + (j + 1) + (size_t)j * n;  // v[0] == 1 implied
+            const int m = n - j - 1;
+            cplx s = x[j + 1];
+            for (int i = 1; i < m; ++i) s += std::conj(v[i]) * x[j + 1 + i];
+            s *= tau[j];
+            x[j + 1] -= s;
+            for (int i = 1; i < m; ++i) x[j + 1 + i] -= s * v[i];
+        }
+    }
+    return true;
+}
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+/* BandStructure.cpp:26-57 */
+int orc_basis(int n, int32_t* G, int capacity) {
+    static const unsigned G2[] = {
+        0,   3,   4,   8,   11,  12,  16,  19,  20,  24,  27,  32,  35,  36,  40,  43,  44,  48,  51,  52,  56,  59,  67,  68,
+        75,  76,  80,  83,  84,  88,  96,  99,  104, 107, 108, 115, 116, 120, 123, 128, 131, 132, 136, 139, 140, 144, 147, 152,
+        155, 160, 163, 164, 168, 171, 172, 176, 179, 180, 184, 187, 192, 195, 196, 200, 203, 204, 208, 211, 212, 216, 219, 224,
+        227, 228, 232, 236, 243, 244, 248, 251, 259, 260, 264, 267, 268, 272, 275, 276, 280, 283, 291, 296, 299, 300, 304, 307,
+        308, 312, 315, 320, 323, 324, 331, 332, 339, 355, 356, 360, 363, 371, 376, 384, 387, 395, 420, 451};
+    const int nG2 = (int)(sizeof(G2) / sizeof(G2[0]));
+    if (n < 2 || n > nG2) return -1;
+    const unsigned cut = G2[n - 1];
+    const int size = (int)std::ceil(std::sqrt((double)cut));
+    int cnt = 0;
+    for (int i = -size; i <= size; ++i)
+        for (int j = -size; j <= size; ++j)
+            for (int k = -size; k <= size; ++k) {
+                const int x = -i + j + k, y = i - j + k, z = i + j - k;  // i*b1 + j*b2 + k*b3
+                const double sq = x * x + y * y + z * z;
+                if (sq <= cut) {
+                    if (cnt < capacity && G) {
+                        G[3 * cnt] = x;
+                        G[3 * cnt + 1] = y;
+                        G[3 * cnt + 2] = z;
+                    }
+                    ++cnt;
+                }
+            }
+    return cnt;
+}
+
+/* SymmetryPoints.cpp:21-147; labels one char each from {L,G,X,W,K,U}.
+ * Returns the number of points written, -1 on invalid input. */
+int orc_kpath(const char* labels, unsigned nrPoints, double* out, unsigned* sympos) {
+    const size_t M = std::strlen(labels);
+    if (M < 2 || nrPoints < M) return -1;
+    std::vector<V3> nodes;
+    for (size_t s = 0; s < M; ++s) {
+        switch (labels[s]) {
+            case 'L': nodes.push_back({0.5, 0.5, 0.5}); break;
+            case 'G': nodes.push_back({0., 0., 0.}); break;
+            case 'X': nodes.push_back({1., 0., 0.}); break;
+            case 'W': nodes.push_back({1., 0.5, 0.}); break;
+            case 'K': nodes.push_back({0.75, 0.75, 0.}); break;
+            case 'U': nodes.push_back({1., 0.25, 0.25}); break;
+            default: return -1;
+        }
+    }
+    std::vector<double> seg(M - 1);
+    for (size_t i = 1; i < M; ++i) seg[i - 1] = length(sub(nodes[i], nodes[i - 1]));
+    const double total = std::accumulate(seg.begin(), seg.end(), 0.0);
+    const unsigned R = nrPoints - (unsigned)M;
+    int cnt = 0;
+    int nsym = 0;
+    auto push = [&](const V3& v) {
+        out[3 * cnt] = v.x;
+        out[3 * cnt + 1] = v.y;
+        out[3 * cnt + 2] = v.z;
+        ++cnt;
+    };
+    if (total == 0.0) {
+        for (size_t j = 0; j < M; ++j) {
+            sympos[nsym++] = cnt;
+            push(nodes[j]);
+        }
+        for (unsigned i = 0; i < R; ++i) push(nodes.back());
+        return cnt;
+    }
+    std::vector<unsigned> m(M - 1);
+    std::vector<double> want(M - 1);
+    std::vector<size_t> order(M - 1);
+    for (size_t s = 0; s < M - 1; ++s) {
+        want[s] = (seg[s] / total) * (double)R;
+        m[s] = (unsigned)std::floor(want[s]);
+        order[s] = s;
+    }
+    const unsigned assigned = std::accumulate(m.begin(), m.end(), 0u);
+    const unsigned rem = R - assigned;
+    std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) {
+        return want[p] - std::floor(want[p]) > want[q] - std::floor(want[q]);
+    });
+    for (unsigned i = 0; i < rem; ++i) ++m[order[i]];
+    sympos[nsym++] = cnt;
+    push(nodes.front());
+    for (size_t s = 0; s + 1 < M; ++s) {
+        const V3& A = nodes[s];
+        const V3 d = sub(nodes[s + 1], A);
+        const double denom = (double)(m[s] + 1);
+        for (unsigned i = 1; i <= m[s]; ++i) {
+            const double t = (double)i / denom;
+            push({A.x + d.x * t, A.y + d.y * t, A.z + d.z * t});
+        }
+        sympos[nsym++] = cnt;
+        push(nodes[s + 1]);
+    }
+    return cnt;
+}
+
+double orc_generalized_bessel(double nu, double x) { return gen_bessel(nu, x); }
+double orc_generalized_bessel_cylindrical(double nu, double x) { return gen_bessel_cyl(nu, x); }
+double orc_i2(double x) { return i2(x); }
+
+/* Single non-local element for reduced K1 = k+G_i, K2 = k+G_j; out = (re, im). */
+void orc_nl_element(const uepm_material* m, const double* K1n, const double* K2n, double* out) {
+    const double a = lattice_m(*m);
+    const V3 tau = {a / 8.0, a / 8.0, a / 8.0};
+    const cplx z = nl_element(*m, {K1n[0], K1n[1], K1n[2]}, {K2n[0], K2n[1], K2n[2]}, tau);
+    out[0] = z.real();
+    out[1] = z.imag();
+}
+
+/* Full dim x dim H(k) (column-major, interleaved re/im) exactly as SetMatrix
+ * leaves it, then hermitised when `herm` != 0.  Returns dim, or -1. */
+int orc_assemble(const uepm_material* m, const int32_t* G, int N, const double* k, int nl, int soc, int herm, double* H) {
+    try {
+        const int dim = soc ? 2 * N : N;
+        assemble(*m, G, N, k, nl != 0, soc != 0, reinterpret_cast<cplx*>(H));
+        if (herm) hermitize(dim, reinterpret_cast<cplx*>(H));
+        return dim;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+/* Dense Hermitian eigen-decomposition (the oracle's stand-in for Eigen). */
+int orc_eigh(int n, const double* H, double* evals, double* evecs) {
+    return eigh(n, reinterpret_cast<const cplx*>(H), evals, reinterpret_cast<cplx*>(evecs)) ? 0 : -1;
+}
+
+/* Tridiagonal reduction only (d, e) — used to check the GPU's T stage. */
+int orc_tridiagonal(int n, const double* H, double* d, double* e) {
+    std::vector<cplx> A(reinterpret_cast<const cplx*>(H), reinterpret_cast<const cplx*>(H) + (size_t)n * n);
+    std::vector<cplx> tau(n);
+    tridiagonalize(n, A.data(), d, e, tau.data());
+    return 0;
+}
+
+/* BandStructure::Compute_parallel (BandStructure.cpp:151-201) over nk
+ * k-points with `nthreads` OpenMP threads, schedule(dynamic); per k:
+ * SetMatrix, Diagonalize (hermitise + eigen-solve), lowest nb eigenvalues,
+ * their eigenvectors (flags & UEPM_VECTORS) and Hellmann-Feynman gradients
+ * (flags & UEPM_GRADIENTS, Hamiltonian.cpp:146-194). */
+int orc_solve(const uepm_material* m, const int32_t* G, int N, unsigned flags, int nb, const double* k, int64_t nk,
+              double* evals, double* evecs, double* grads, int nthreads) {
+    const bool nl = flags & UEPM_NONLOCAL, soc = flags & UEPM_SOC;
+    const bool grad = flags & UEPM_GRADIENTS;
+    const bool vec = (flags & UEPM_VECTORS) || grad;
+    const int dim = soc ? 2 * N : N;
+    if (nb < 1 || nb > dim) {
+        g_err = "nb out of range";
+        return -1;
+    }
+    int status = 0;
+    const double a = lattice_m(*m);
+    const double tpa = 2.0 * kPi / a;
+    const double diag_factor = (kHbar * kHbar) / (2.0 * kMe * kQe);
+    const double pref = 2.0 * diag_factor * (tpa * tpa);
+#pragma omp parallel num_threads(nthreads)
+    {
+        std::vector<cplx> H((size_t)dim * dim), V(vec ? (size_t)dim * dim : 0);
+        std::vector<double> w(dim);
+#pragma omp for schedule(dynamic)
+        for (int64_t ik = 0; ik < nk; ++ik) {
+            try {
+                assemble(*m, G, N, k + 3 * ik, nl, soc, H.data());
+                hermitize(dim, H.data());
+                if (!eigh(dim, H.data(), w.data(), vec ? V.data() : nullptr)) throw std::runtime_error("Eigenvalue decomposition failed");
+            } catch (const std::exception& ex) {
+#pragma omp critical
+                {
+                    g_err = ex.what();
+                    status = -1;
+                }
+                continue;
+            }
+            for (int b = 0; b < nb; ++b) {
+                evals[ik * nb + b] = w[b];
+                if (evecs && vec) std::memcpy(evecs + 2 * ((ik * nb + b) * (int64_t)dim), V.data() + (size_t)b * dim, sizeof(cplx) * dim);
+                if (grads && grad) {
+                    const cplx* ev = V.data() + (size_t)b * dim;
+                    double sx = 0, sy = 0, sz = 0;
+                    for (int i = 0; i < N; ++i) {
+                        const double wt = soc ? std::norm(ev[i]) + std::norm(ev[i + N]) : std::norm(ev[i]);
+                        sx += (k[3 * ik] + G[3 * i]) * wt;
+                        sy += (k[3 * ik + 1] + G[3 * i + 1]) * wt;
+                        sz += (k[3 * ik + 2] + G[3 * i + 2]) * wt;
+                    }
+                    const double f2p = 1.0 / tpa;
+                    grads[3 * (ik * nb + b)] = sx * pref * f2p;
+                    grads[3 * (ik * nb + b) + 1] = sy * pref * f2p;
+                    grads[3 * (ik * nb + b) + 2] = sz * pref * f2p;
+                }
+            }
+        }
+    }
+    return status;
+}
+
+/* Conditioning of the reference's F_l general branch at one k: the smallest
+ * | |K_i| - |K_j| | (1/m) over pairs with | |K_i| - |K_j| | > 1e-4, i.e. the
+ * pairs whose element carries rounding noise ~ eps*|K| / that difference
+ * (Material.cpp:95-99). */
+double orc_min_norm_gap(const uepm_material* m, const int32_t* G, int N, const double* k) {
+    const double fourier = 2.0 * kPi / lattice_m(*m);
+    std::vector<double> nk(N);
+    for (int i = 0; i < N; ++i)
+        nk[i] = length(scale(V3{k[0] + G[3 * i], k[1] + G[3 * i + 1], k[2] + G[3 * i + 2]}, fourier));
+    double best = 1e300;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const double g = std::fabs(nk[i] - nk[j]);
+            if (g > 1.0e-4 && g < best) best = g;
+        }
+    return best;
+}
+
+/* is_in_first_BZ (DielectricFunction.cpp:35-40) */
+int orc_in_first_bz(const double* k) {
+    return std::fabs(k[0]) <= 1.0 && std::fabs(k[1]) <= 1.0 && std::fabs(k[2]) <= 1.0 &&
+           std::fabs(k[0]) + std::fabs(k[1]) + std::fabs(k[2]) <= 3.0 / 2.0;
+}
+
+/* FindBandGap + AdjustValues (BandStructure.cpp:203-262) on a k-major
+ * nk x nb table, in place; returns the band gap. */
+double orc_adjust_values(double* E, int64_t nk, int nb, int soc, int min_cond_to_zero) {
+    double maxV = 0, minC = 0, gap = 0;
+    bool found = false;
+    if (nk > 0 && nb >= 2) {
+        double fallback = -1.7976931348623157e308;
+        maxV = fallback;
+        for (int lo = 2; lo + 1 < nb; ++lo) {
+            maxV = -1.7976931348623157e308;
+            minC = INFINITY;
+            for (int64_t i = 0; i < nk; ++i) {
+                maxV = std::max(maxV, E[i * nb + lo]);
+                minC = std::min(minC, E[i * nb + lo + 1]);
+            }
+            if (lo == 3) fallback = maxV;
+            if (maxV + 0.35 < minC) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) maxV = fallback;
+    }
+    if (found) gap = minC - maxV;
+    const int nval = soc ? 8 : 4;
+    for (int64_t i = 0; i < nk; ++i)
+        for (int b = 0; b < nb; ++b) {
+            if (b < nval) E[i * nb + b] -= maxV;
+            else if (min_cond_to_zero) E[i * nb + b] -= minC;
+            else E[i * nb + b] -= maxV;
+        }
+    return gap;
+}
+
+double orc_ryd_to_ev(void) { return kRydEV; }
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+"""ctypes view of the C ABI declared in include/uepm.h.
+
+The shared library ``libuepm.so`` (HIP kernels + host C++) is built in-tree
+by ``ultimateepm_amd.build``.  Loading fails loudly when it is missing: there
+is no CPU fallback for the band solve.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libuepm.so")
+
+UEPM_OK = 0
+UEPM_ERR_INVALID_ARG = 1
+UEPM_ERR_RUNTIME = 2
+UEPM_ERR_OUT_OF_RANGE = 3
+UEPM_ERR_DEVICE = 4
+UEPM_ERR_NOT_CONVERGED = 5
+UEPM_ERR_NO_MEMORY = 6
+
+NONLOCAL = 0x1
+SOC = 0x2
+VECTORS = 0x4
+GRADIENTS = 0x8
+MIXED = 0x10
+
+
+class uepm_material(C.Structure):
+    """Mirror of ``uepm_material`` (include/uepm.h)."""
+
+    _fields_ = [
+        ("lattice_constant_angstrom", C.c_double),
+        ("V_S", C.c_double * 4),
+        ("V_A", C.c_double * 4),
+        ("nl_populated", C.c_int32),
+        ("nl_well_type", C.c_int32),
+        ("alpha_0_cation", C.c_double),
+        ("beta_0_cation", C.c_double),
+        ("A2_cation", C.c_double),
+        ("alpha_0_anion", C.c_double),
+        ("beta_0_anion", C.c_double),
+        ("A2_anion", C.c_double),
+        ("R0_cation", C.c_double),
+        ("R2_cation", C.c_double),
+        ("R0_anion", C.c_double),
+        ("R2_anion", C.c_double),
+        ("soc_populated", C.c_int32),
+        ("period_anion", C.c_int32),
+        ("period_cation", C.c_int32),
+        ("radial_extent_anion", C.c_double),
+        ("radial_extent_cation", C.c_double),
+        ("alpha_soc", C.c_double),
+        ("mu_soc", C.c_double),
+    ]
+
+
+class UepmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"uepm error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libuepm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(the band solve has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
+    L.uepm_last_error.restype = C.c_char_p
+    L.uepm_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+    L.uepm_ctx_destroy.argtypes = [P]
+    L.uepm_basis_vectors.argtypes = [C.c_int, C.POINTER(I32), C.c_int, C.POINTER(C.c_int)]
+    L.uepm_kpath.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_double), C.POINTER(C.c_uint), C.POINTER(C.c_int)]
+    L.uepm_bz_sample.argtypes = [C.c_uint64, I64, C.POINTER(C.c_double)]
+    L.uepm_plan_create.argtypes = [P, C.POINTER(uepm_material), C.POINTER(I32), C.c_int, C.c_uint, C.c_int, C.POINTER(P)]
+    L.uepm_plan_destroy.argtypes = [P]
+    L.uepm_plan_dim.argtypes = [P]
+    L.uepm_solve.argtypes = [P, D, I64, D, D, D]
+    L.uepm_solve_device.argtypes = [P, P, I64, P, P, P, P]
+    L.uepm_assemble_device.argtypes = [P, P, I64, P, P]
+    L.uepm_plan_set_timing.argtypes = [P, C.c_int]
+    L.uepm_plan_last_timing.argtypes = [P, C.c_int, D, C.POINTER(I64), C.POINTER(C.c_char_p)]
+    U64 = C.c_uint64
+    L.uepm_device_alloc.argtypes = [P, U64, C.POINTER(P)]
+    L.uepm_device_free.argtypes = [P, P]
+    L.uepm_memcpy_htod.argtypes = [P, P, P, U64]
+    L.uepm_memcpy_dtoh.argtypes = [P, P, P, U64]
+    L.uepm_memset_device.argtypes = [P, P, C.c_int, U64]
+    L.uepm_stream_create.argtypes = [P, C.POINTER(P)]
+    L.uepm_stream_destroy.argtypes = [P, P]
+    L.uepm_stream_synchronize.argtypes = [P, P]
+    L.uepm_device_synchronize.argtypes = [P]
+    _lib = L
+    return L
+
+
+def check(code: int) -> None:
+    if code != UEPM_OK:
+        raise UepmError(code, lib().uepm_last_error().decode())

@@ -1,0 +1,523 @@
+// solver_f64.hip — FP64 (complex128) band solve stages on gfx950.
+//
+//   (a) assemble_kernel     H(k) for a batch of k-points (Hamiltonian::SetMatrix +
+//                           the 0.5 (H + H^dagger) step of Diagonalize,
+//                           src/EPP/Hamiltonian.cpp:50-127)
+//   (b) tridiag_kernel      Householder reduction H = Q T Q^H, T real symmetric
+//                           tridiagonal, one workgroup per k
+//   (c) tri_eig_kernel      bisection (Sturm counts) for the lowest nb eigenvalues
+//                           of T and inverse iteration for their vectors, one lane
+//                           per (k, band)
+//   (d) backtransform_kernel  cluster re-orthogonalisation, x = Q z, unit norm,
+//                           Hellmann-Feynman gradients (Hamiltonian.cpp:146-194)
+//
+// (b)-(d) replace Eigen::SelfAdjointEigenSolver::compute (Hamiltonian.cpp:129):
+// same eigenvalues to within the backward-stability envelope of a Householder
+// reduction, eigenvectors up to phase (and basis inside degenerate clusters).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "epm_physics.hpp"
+#include "solver.hpp"
+
+namespace uepm {
+
+// ---------------------------------------------------------------- helpers
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup; every thread gets the result. `scratch` holds
+// BS/64 doubles; a __syncthreads() precedes reuse of `scratch`.
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* scratch) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) s += scratch[i];
+    return s;
+}
+
+// ------------------------------------------------------------ (a) assembly
+// One workgroup per k.  Per-basis-vector state in LDS, then every (i, j) pair
+// of the lower triangle: h_ij and h_ji are both evaluated as the reference
+// evaluates them and the stored value is 0.5 (h_ij + conj(h_ji)) (its
+// Diagonalize hermitisation), mirrored into the upper triangle.
+// Output: column-major dim x dim, leading dimension ld, one matrix per k.
+template <int BS>
+__global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t* __restrict__ G,
+                                                       const double* __restrict__ kpts, double2* __restrict__ H,
+                                                       int ld) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    KState* ks = reinterpret_cast<KState*>(smem);
+    const int kk = blockIdx.x;
+    const double* k = kpts + 3 * (size_t)kk;
+    const int N = P.N;
+    for (int i = threadIdx.x; i < N; i += BS) kstate_init(P, k, G, i, ks[i]);
+    __syncthreads();
+    double2* A = H + (size_t)kk * ld * P.dim;
+    const int npairs = N * (N + 1) / 2;
+    for (int t = threadIdx.x; t < npairs; t += BS) {
+        // t -> (i, j), i >= j, column-major over the packed lower triangle
+        int j = (int)((2.0 * N + 1.0 - sqrt((2.0 * N + 1.0) * (2.0 * N + 1.0) - 8.0 * t)) * 0.5);
+        while (j > 0 && j * (2 * N - j + 1) / 2 > t) --j;
+        while ((j + 1) * (2 * N - j) / 2 <= t) ++j;
+        const int i = j + (t - j * (2 * N - j + 1) / 2);
+        const KState& si = ks[i];
+        const KState& sj = ks[j];
+        const int gx = G[3 * i] - G[3 * j], gy = G[3 * i + 1] - G[3 * j + 1], gz = G[3 * i + 2] - G[3 * j + 2];
+        double re_ij, im_ij, re_ji, im_ji;
+        local_V(P, gx, gy, gz, re_ij, im_ij);
+        local_V(P, -gx, -gy, -gz, re_ji, im_ji);
+        if (i == j) {
+            const double KG2 = P.kin * (si.kx * si.kx + si.ky * si.ky + si.kz * si.kz);
+            re_ij += KG2;
+            re_ji += KG2;
+        }
+        if (P.nl) {
+            double r1, i1, r2, i2;
+            nonlocal(P, si, sj, r1, i1);
+            nonlocal(P, sj, si, r2, i2);
+            re_ij += r1; im_ij += i1;
+            re_ji += r2; im_ji += i2;
+        }
+        if (!P.soc) {
+            const double hr = 0.5 * (re_ij + re_ji), hi = 0.5 * (im_ij - im_ji);
+            A[i + (size_t)j * ld] = make_double2(hr, hi);
+            A[j + (size_t)i * ld] = make_double2(hr, -hi);
+        } else {
+            double s_ij[8], s_ji[8];
+            soc_block(P, si, sj, gx, gy, gz, s_ij);
+            soc_block(P, sj, si, -gx, -gy, -gz, s_ji);
+            // blocks: [[M+S00, S01],[S10, M+S11]]; element (row, col)
+            // (i, j) of block (p, q) pairs with (j, i) of block (q, p).
+            const int n0 = N;
+            // up-up
+            {
+                const double ar = re_ij + s_ij[0], ai = im_ij + s_ij[1];
+                const double br = re_ji + s_ji[0], bi = im_ji + s_ji[1];
+                const double hr = 0.5 * (ar + br), hi = 0.5 * (ai - bi);
+                A[i + (size_t)j * ld] = make_double2(hr, hi);
+                A[j + (size_t)i * ld] = make_double2(hr, -hi);
+            }
+            // down-down
+            {
+                const double ar = re_ij + s_ij[6], ai = im_ij + s_ij[7];
+                const double br = re_ji + s_ji[6], bi = im_ji + s_ji[7];
+                const double hr = 0.5 * (ar + br), hi = 0.5 * (ai - bi);
+                A[(i + n0) + (size_t)(j + n0) * ld] = make_double2(hr, hi);
+                A[(j + n0) + (size_t)(i + n0) * ld] = make_double2(hr, -hi);
+            }
+            // down-up (row i+N, col j) pairs with up-down (row j, col i+N)
+            {
+                const double hr = 0.5 * (s_ij[4] + s_ji[2]), hi = 0.5 * (s_ij[5] - s_ji[3]);
+                A[(i + n0) + (size_t)j * ld] = make_double2(hr, hi);
+                A[j + (size_t)(i + n0) * ld] = make_double2(hr, -hi);
+            }
+            if (i != j) {
+                // down-up (row j+N, col i) pairs with up-down (row i, col j+N)
+                const double hr = 0.5 * (s_ji[4] + s_ij[2]), hi = 0.5 * (s_ji[5] - s_ij[3]);
+                A[(j + n0) + (size_t)i * ld] = make_double2(hr, hi);
+                A[i + (size_t)(j + n0) * ld] = make_double2(hr, -hi);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------- (b) tridiagonalisation
+// Unblocked Householder on the full Hermitian matrix held column-major in
+// device memory.  Step j: reflector H_j = I - tau v v^H annihilating
+// A[j+2:, j]; p = tau A22 v; w = p - (tau/2)(p^H v) v; A22 -= v w^H + w v^H.
+// v (v[0] = 1 implied) overwrites A[j+2:, j]; outputs d, e (real), tau.
+template <int BS>
+__global__ __launch_bounds__(BS) void tridiag_kernel(double2* __restrict__ H, int n, int ld, double* __restrict__ dout,
+                                                     double* __restrict__ eout, double2* __restrict__ tout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double2* v = reinterpret_cast<double2*>(smem);
+    double2* w = v + n;
+    double* red = reinterpret_cast<double*>(w + n);
+    __shared__ double2 s_tau;
+    const int kk = blockIdx.x;
+    double2* A = H + (size_t)kk * ld * n;
+    double* d = dout + (size_t)kk * n;
+    double* e = eout + (size_t)kk * n;
+    double2* tau = tout + (size_t)kk * n;
+    const int tid = threadIdx.x;
+
+    for (int j = 0; j + 1 < n; ++j) {
+        double2* colj = A + (size_t)j * ld;
+        const double2 alpha = colj[j + 1];  // read before the barriers in block_sum
+        double part = 0.0;
+        for (int r = j + 2 + tid; r < n; r += BS) {
+            const double2 x = colj[r];
+            part += x.x * x.x + x.y * x.y;
+        }
+        const double xn2 = block_sum<BS>(part, red);
+        double2 tj = make_double2(0.0, 0.0);
+        double beta = alpha.x;
+        double2 sc = make_double2(1.0, 0.0);
+        if (!(xn2 == 0.0 && alpha.y == 0.0)) {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            beta = alpha.x >= 0 ? -nrm : nrm;
+            tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            // sc = 1 / (alpha - beta)
+            const double ar = alpha.x - beta, ai = alpha.y;
+            const double den = ar * ar + ai * ai;
+            sc = make_double2(ar / den, -ai / den);
+        }
+        for (int r = j + 2 + tid; r < n; r += BS) {
+            const double2 x = cmul(colj[r], sc);
+            colj[r] = x;
+            v[r] = x;
+        }
+        if (tid == 0) {
+            v[j + 1] = make_double2(1.0, 0.0);
+            colj[j + 1] = make_double2(beta, 0.0);
+            e[j] = beta;
+            tau[j] = tj;
+            d[j] = A[j + (size_t)j * ld].x;
+        }
+        __syncthreads();
+        if (tj.x == 0.0 && tj.y == 0.0) continue;
+        // p = tau * A22 v
+        for (int r = j + 1 + tid; r < n; r += BS) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int c = j + 1; c < n; ++c) {
+                const double2 a = A[r + (size_t)c * ld];
+                const double2 vc = v[c];
+                acc.x += a.x * vc.x - a.y * vc.y;
+                acc.y += a.x * vc.y + a.y * vc.x;
+            }
+            w[r] = cmul(tj, acc);
+        }
+        __syncthreads();
+        double pr = 0.0, pi = 0.0;
+        for (int r = j + 1 + tid; r < n; r += BS) {
+            const double2 q = cmulc(w[r], v[r]);
+            pr += q.x;
+            pi += q.y;
+        }
+        pr = block_sum<BS>(pr, red);
+        __syncthreads();
+        pi = block_sum<BS>(pi, red);
+        // al = -0.5 * tau * (p^H v)
+        const double2 al = cmul(make_double2(-0.5 * tj.x, -0.5 * tj.y), make_double2(pr, pi));
+        for (int r = j + 1 + tid; r < n; r += BS) w[r] = cadd(w[r], cmul(al, v[r]));
+        __syncthreads();
+        // A22 -= v w^H + w v^H
+        for (int r = j + 1 + tid; r < n; r += BS) {
+            const double2 vr = v[r], wr = w[r];
+            for (int c = j + 1; c < n; ++c) {
+                const double2 vc = v[c], wc = w[c];
+                double2 a = A[r + (size_t)c * ld];
+                // v_r conj(w_c) + w_r conj(v_c)
+                a.x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
+                a.y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
+                A[r + (size_t)c * ld] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[n - 1] = A[(n - 1) + (size_t)(n - 1) * ld].x;
+        e[n - 1] = 0.0;
+        tau[n - 1] = make_double2(0.0, 0.0);
+    }
+}
+
+// ------------------------------------- (c) tridiagonal eigenvalues / vectors
+// One lane per (k, band).  Per-lane work arrays are interleaved across lanes
+// (element i of lane t at W[i * T + t]) so that lock-step lanes coalesce.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e, int n, double x, double pivmin) {
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int c = q < 0.0;
+    for (int i = 1; i < n; ++i) {
+        q = (d[i] - x) - (e[i - 1] * e[i - 1]) / q;
+        if (fabs(q) < pivmin) q = -pivmin;
+        c += q < 0.0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double lane_rand(uint32_t i, uint32_t b) {
+    uint32_t h = i * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (double)(h >> 8) * (1.0 / 8388608.0) - 1.0;
+}
+
+__global__ void tri_eig_kernel(const double* __restrict__ dall, const double* __restrict__ eall, int n, int nb,
+                               int64_t nk, double* __restrict__ evals, double* __restrict__ Z, double* __restrict__ W,
+                               int want_vectors) {
+    const int64_t T = nk * nb;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const int64_t kk = t / nb;
+    const int b = (int)(t % nb);
+    const double* d = dall + kk * n;
+    const double* e = eall + kk * n;
+    // Gershgorin interval and scale
+    double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0, tnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+        const double er = i + 1 < n ? fabs(e[i]) : 0.0;
+        lo = fmin(lo, d[i] - el - er);
+        hi = fmax(hi, d[i] + el + er);
+        emax2 = fmax(emax2, er * er);
+        tnorm = fmax(tnorm, fabs(d[i]) + el + er);
+    }
+    const double pivmin = DBL_MIN * fmax(1.0, emax2);
+    const double eps = DBL_EPSILON;
+    lo -= 2.0 * eps * tnorm + pivmin;
+    hi += 2.0 * eps * tnorm + pivmin;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || mid <= lo || mid >= hi) break;
+        if (sturm_count(d, e, n, mid, pivmin) > b) hi = mid;
+        else lo = mid;
+    }
+    const double lam = 0.5 * (lo + hi);
+    evals[kk * nb + b] = lam;
+    if (!want_vectors) return;
+
+    // Inverse iteration with an LU factorisation of T - lam I with partial
+    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|).
+    double* L = W;                 // multipliers
+    double* U0 = W + (size_t)n * T;      // pivots
+    double* U1 = W + (size_t)2 * n * T;  // first super-diagonal
+    double* U2 = W + (size_t)3 * n * T;  // second super-diagonal
+    double* PV = W + (size_t)4 * n * T;  // interchange flags
+    double* X = W + (size_t)5 * n * T;   // iterate
+#define IDX(i) ((size_t)(i) * T + t)
+    const double tiny = eps * tnorm;
+    {
+        double a = d[0] - lam;
+        double c = n > 1 ? e[0] : 0.0;
+        for (int k = 0; k + 1 < n; ++k) {
+            const double sub = e[k];
+            const double dn = d[k + 1] - lam;
+            const double en = (k + 2 < n) ? e[k + 1] : 0.0;
+            if (fabs(a) >= fabs(sub)) {
+                if (a == 0.0) a = tiny;
+                const double mult = sub / a;
+                L[IDX(k)] = mult;
+                U0[IDX(k)] = a;
+                U1[IDX(k)] = c;
+                U2[IDX(k)] = 0.0;
+                PV[IDX(k)] = 0.0;
+                a = dn - mult * c;
+                c = en;
+            } else {
+                const double mult = a / sub;
+                L[IDX(k)] = mult;
+                U0[IDX(k)] = sub;
+                U1[IDX(k)] = dn;
+                U2[IDX(k)] = en;
+                PV[IDX(k)] = 1.0;
+                a = c - mult * dn;
+                c = -mult * en;
+            }
+        }
+        if (a == 0.0) a = tiny;
+        U0[IDX(n - 1)] = a;
+    }
+    for (int i = 0; i < n; ++i) X[IDX(i)] = lane_rand((uint32_t)i, (uint32_t)b);
+    for (int iter = 0; iter < 3; ++iter) {
+        // forward: apply interchanges and L
+        for (int k = 0; k + 1 < n; ++k) {
+            double yk = X[IDX(k)], yk1 = X[IDX(k + 1)];
+            if (PV[IDX(k)] != 0.0) {
+                const double tmp = yk;
+                yk = yk1;
+                yk1 = tmp;
+            }
+            yk1 -= L[IDX(k)] * yk;
+            X[IDX(k)] = yk;
+            X[IDX(k + 1)] = yk1;
+        }
+        // back substitution
+        double x2 = 0.0, x1 = X[IDX(n - 1)] / U0[IDX(n - 1)];
+        X[IDX(n - 1)] = x1;
+        double amax = fabs(x1);
+        for (int k = n - 2; k >= 0; --k) {
+            const double xk = (X[IDX(k)] - U1[IDX(k)] * x1 - U2[IDX(k)] * x2) / U0[IDX(k)];
+            X[IDX(k)] = xk;
+            x2 = x1;
+            x1 = xk;
+            amax = fmax(amax, fabs(xk));
+        }
+        const double s = 1.0 / amax;
+        for (int i = 0; i < n; ++i) X[IDX(i)] *= s;
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) nrm += X[IDX(i)] * X[IDX(i)];
+    const double s = 1.0 / sqrt(nrm);
+    double* z = Z + (size_t)t * n;
+    for (int i = 0; i < n; ++i) z[i] = X[IDX(i)] * s;
+#undef IDX
+}
+
+// ------------------------------------------- (d) back-transform + gradients
+// One workgroup per k, 4 waves; wave w owns bands b = w, w+4, ...
+// Z: nk x nb x n real eigenvectors of T (unit norm).  X (output, complex):
+// nk x nb x dim.  Bands within ortol of each other are re-orthogonalised
+// (modified Gram-Schmidt, ascending) before x = Q z.
+template <int BS>
+__global__ __launch_bounds__(BS) void backtransform_kernel(const double2* __restrict__ H, int n, int ld,
+                                                           const double2* __restrict__ tauall,
+                                                           const double* __restrict__ dall,
+                                                           const double* __restrict__ eall,
+                                                           const double* __restrict__ evals, const double* Z,
+                                                           double2* __restrict__ X, double* __restrict__ grads,
+                                                           DevParams P, const double* __restrict__ kpts,
+                                                           const int32_t* __restrict__ G, int nb) {
+    const int kk = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    constexpr int NW = BS / 64;
+    const double2* A = H + (size_t)kk * ld * n;
+    const double2* tau = tauall + (size_t)kk * n;
+    const double* z = Z + (size_t)kk * nb * n;
+    double2* x = X + (size_t)kk * nb * n;
+    const double* ev = evals + (size_t)kk * nb;
+    // ||T||_1 for the cluster tolerance
+    double tn = 0.0;
+    {
+        const double* d = dall + (size_t)kk * n;
+        const double* e = eall + (size_t)kk * n;
+        for (int i = lane; i < n; i += 64) {
+            const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+            const double er = i + 1 < n ? fabs(e[i]) : 0.0;
+            tn = fmax(tn, fabs(d[i]) + el + er);
+        }
+        for (int o = 32; o > 0; o >>= 1) tn = fmax(tn, __shfl_xor(tn, o, 64));
+    }
+    const double ortol = 1e-3 * tn;
+    // Copy z -> x (real), MGS within clusters.  Serial over bands in wave 0.
+    if (wv == 0) {
+        int cstart = 0;
+        for (int b = 0; b < nb; ++b) {
+            if (b > 0 && ev[b] - ev[b - 1] > ortol) cstart = b;
+            double2* xb = x + (size_t)b * n;
+            for (int i = lane; i < n; i += 64) xb[i] = make_double2(z[(size_t)b * n + i], 0.0);
+            for (int c = cstart; c < b; ++c) {
+                const double2* xc = x + (size_t)c * n;
+                double s = 0.0;
+                for (int i = lane; i < n; i += 64) s += xc[i].x * xb[i].x;
+                s = wave_sum(s);
+                for (int i = lane; i < n; i += 64) xb[i].x -= s * xc[i].x;
+            }
+            double nn = 0.0;
+            for (int i = lane; i < n; i += 64) nn += xb[i].x * xb[i].x;
+            nn = wave_sum(nn);
+            const double inv = 1.0 / sqrt(nn);
+            for (int i = lane; i < n; i += 64) xb[i].x *= inv;
+        }
+    }
+    __syncthreads();
+    // x = H_0 H_1 ... H_{n-2} z: apply H_{n-2} first.
+    for (int b = wv; b < nb; b += NW) {
+        double2* xb = x + (size_t)b * n;
+        for (int j = n - 2; j >= 0; --j) {
+            const double2 tj = tau[j];
+            if (tj.x == 0.0 && tj.y == 0.0) continue;
+            const double2* vj = A + (size_t)j * ld;  // v at rows j+1..n-1, v[j+1] = 1
+            const int r0 = lane + (j + 1 > lane ? ((j + 1 - lane + 63) >> 6) << 6 : 0);
+            double sr = 0.0, si = 0.0;
+            for (int r = r0; r < n; r += 64) {
+                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : vj[r];
+                const double2 q = cmulc(vr, xb[r]);
+                sr += q.x;
+                si += q.y;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            const double2 s = cmul(tj, make_double2(sr, si));
+            for (int r = r0; r < n; r += 64) {
+                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : vj[r];
+                xb[r] = csub(xb[r], cmul(s, vr));
+            }
+        }
+        __threadfence_block();  // rows i + N were written by other lanes
+        if (grads) {
+            // Hellmann-Feynman on the kinetic term (Hamiltonian.cpp:146-194)
+            const double* k = kpts + 3 * (size_t)kk;
+            const int N = P.N;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int i = lane; i < N; i += 64) {
+                double wt = xb[i].x * xb[i].x + xb[i].y * xb[i].y;
+                if (P.soc) wt += xb[i + N].x * xb[i + N].x + xb[i + N].y * xb[i + N].y;
+                sx += (k[0] + (double)G[3 * i]) * wt;
+                sy += (k[1] + (double)G[3 * i + 1]) * wt;
+                sz += (k[2] + (double)G[3 * i + 2]) * wt;
+            }
+            sx = wave_sum(sx);
+            sy = wave_sum(sy);
+            sz = wave_sum(sz);
+            if (lane == 0) {
+                double* g = grads + 3 * ((size_t)kk * nb + b);
+                g[0] = sx * P.grad_pref * P.frac_to_phys;
+                g[1] = sy * P.grad_pref * P.frac_to_phys;
+                g[2] = sz * P.grad_pref * P.frac_to_phys;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ launchers
+constexpr int kAsmBS = 256;
+constexpr int kTriBS = 256;
+constexpr int kBtBS = 256;
+
+hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
+                           hipStream_t s) {
+    const size_t shm = sizeof(KState) * (size_t)P.N;
+    hipLaunchKernelGGL(assemble_kernel<kAsmBS>, dim3((unsigned)nk), dim3(kAsmBS), shm, s, P, dG, dk, dH, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, double* de, double2* dtau,
+                          hipStream_t s) {
+    const size_t shm = sizeof(double2) * 2 * (size_t)n + sizeof(double) * (kTriBS / 64);
+    hipLaunchKernelGGL(tridiag_kernel<kTriBS>, dim3((unsigned)nk), dim3(kTriBS), shm, s, dH, n, ld, dd, de, dtau);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
+                          double* dW, int want_vectors, hipStream_t s) {
+    const int64_t T = nk * nb;
+    const int bs = 64;
+    hipLaunchKernelGGL(tri_eig_kernel, dim3((unsigned)((T + bs - 1) / bs)), dim3(bs), 0, s, dd, de, n, nb, nk, devals,
+                       dZ, dW, want_vectors);
+    return hipGetLastError();
+}
+
+hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2* dtau, const double* dd,
+                                const double* de, const double* devals, const double* dZ, double2* dX, double* dgrads,
+                                const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(backtransform_kernel<kBtBS>, dim3((unsigned)nk), dim3(kBtBS), 0, s, dH, n, ld, dtau, dd, de,
+                       devals, dZ, dX, dgrads, P, dk, dG, nb);
+    return hipGetLastError();
+}
+
+}  // namespace uepm

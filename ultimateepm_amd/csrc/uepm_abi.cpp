@@ -1,0 +1,561 @@
+// uepm_abi.cpp — host side of the C ABI (include/uepm.h).
+//
+// Replaces the reference's per-thread Hamiltonian loop
+// (BandStructure::Compute_parallel, src/EPP/BandStructure.cpp:151-201) by
+// batched device stages; also restates the host-only helpers the path needs
+// (basis generation BandStructure.cpp:26-57, k path SymmetryPoints.cpp:47-147,
+// BZ sampling DielectricFunction.cpp:35-57).  No exception crosses the ABI:
+// errors become status codes plus a per-thread message (uepm_last_error).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/uepm.h"
+#include "solver.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                             \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(UEPM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// src/Common/physical_constants.hpp:18-41, same operation order.
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kH = 6.62607015e-34;
+constexpr double kQe = 1.602176634e-19;
+constexpr double kHbar = kH / (2.0 * kPi);
+constexpr double kMe = 9.1093837015e-31;
+constexpr double kBohr = 5.29177210903e-11;
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kNumStages };
+const char* kStageNames[kNumStages] = {"assemble_kernel", "tridiag_kernel", "tri_eig_kernel", "backtransform_kernel"};
+
+}  // namespace
+
+struct uepm_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+};
+
+struct uepm_plan {
+    uepm_ctx* ctx = nullptr;
+    uepm::DevParams P{};
+    unsigned flags = 0;
+    int N = 0, dim = 0, nb = 0;
+    DeviceBuffer G, H, d, e, tau, Z, W, X;
+    int timing = 0;
+    std::vector<hipEvent_t> ev;  // pairs per stage launch
+    double stage_ms[kNumStages] = {0, 0, 0, 0};
+    int64_t stage_launches[kNumStages] = {0, 0, 0, 0};
+    ~uepm_plan() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+extern "C" {
+
+const char* uepm_last_error(void) { return g_err.c_str(); }
+
+int uepm_ctx_create(int device, uepm_ctx** out) {
+    if (!out) return fail(UEPM_ERR_INVALID_ARG, "uepm_ctx_create: null output");
+    int n = 0;
+    HIP_OK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(UEPM_ERR_INVALID_ARG, "uepm_ctx_create: no such device");
+    auto* c = new uepm_ctx;
+    c->device = device;
+    hipError_t e = hipGetDeviceProperties(&c->prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(UEPM_ERR_DEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return UEPM_OK;
+}
+
+int uepm_ctx_destroy(uepm_ctx* ctx) {
+    delete ctx;
+    return UEPM_OK;
+}
+
+// BandStructure::GenerateBasisVectors (src/EPP/BandStructure.cpp:26-57)
+int uepm_basis_vectors(int nn, int32_t* G, int capacity, int* count) {
+    static const unsigned G2[] = {
+        0,   3,   4,   8,   11,  12,  16,  19,  20,  24,  27,  32,  35,  36,  40,  43,  44,  48,  51,  52,  56,  59,  67,  68,
+        75,  76,  80,  83,  84,  88,  96,  99,  104, 107, 108, 115, 116, 120, 123, 128, 131, 132, 136, 139, 140, 144, 147, 152,
+        155, 160, 163, 164, 168, 171, 172, 176, 179, 180, 184, 187, 192, 195, 196, 200, 203, 204, 208, 211, 212, 216, 219, 224,
+        227, 228, 232, 236, 243, 244, 248, 251, 259, 260, 264, 267, 268, 272, 275, 276, 280, 283, 291, 296, 299, 300, 304, 307,
+        308, 312, 315, 320, 323, 324, 331, 332, 339, 355, 356, 360, 363, 371, 376, 384, 387, 395, 420, 451};
+    const int nG2 = (int)(sizeof(G2) / sizeof(G2[0]));
+    if (nn < 2 || nn > nG2)
+        return fail(UEPM_ERR_RUNTIME, "BandStructure::Initialize: GenerateBasisVectors failed (nearestNeighborsNumber must be between 2 and " +
+                                          std::to_string(nG2) + ")");
+    const unsigned cut = G2[nn - 1];
+    const int size = (int)std::ceil(std::sqrt((double)cut));
+    int cnt = 0;
+    for (int i = -size; i <= size; ++i)
+        for (int j = -size; j <= size; ++j)
+            for (int k = -size; k <= size; ++k) {
+                const int x = -i + j + k, y = i - j + k, z = i + j - k;
+                if ((double)(x * x + y * y + z * z) <= cut) {
+                    if (G && cnt < capacity) {
+                        G[3 * cnt] = x;
+                        G[3 * cnt + 1] = y;
+                        G[3 * cnt + 2] = z;
+                    }
+                    ++cnt;
+                }
+            }
+    if (count) *count = cnt;
+    return UEPM_OK;
+}
+
+// SymmetryPoints::GeneratePoints (src/EPP/SymmetryPoints.cpp:47-147)
+int uepm_kpath(const char* labels, unsigned nrPoints, double* out, unsigned* sympos, int* count) {
+    if (!labels) return fail(UEPM_ERR_INVALID_ARG, "null path");
+    const size_t M = std::strlen(labels);
+    if (M < 2) return fail(UEPM_ERR_INVALID_ARG, "Path must contain at least two symmetry points.");
+    if (nrPoints < M) return fail(UEPM_ERR_INVALID_ARG, "nrPoints must be at least the number of nodes in the path.");
+    struct P3 {
+        double x, y, z;
+    };
+    std::vector<P3> nodes;
+    for (size_t s = 0; s < M; ++s) {
+        switch (labels[s]) {
+            case 'L': nodes.push_back({0.5, 0.5, 0.5}); break;
+            case 'G': nodes.push_back({0., 0., 0.}); break;
+            case 'X': nodes.push_back({1., 0., 0.}); break;
+            case 'W': nodes.push_back({1., 0.5, 0.}); break;
+            case 'K': nodes.push_back({0.75, 0.75, 0.}); break;
+            case 'U': nodes.push_back({1., 0.25, 0.25}); break;
+            default: return fail(UEPM_ERR_INVALID_ARG, std::string("Unknown symmetry point label: ") + labels[s]);
+        }
+    }
+    std::vector<double> seg(M - 1);
+    for (size_t i = 1; i < M; ++i) {
+        const double dx = nodes[i].x - nodes[i - 1].x, dy = nodes[i].y - nodes[i - 1].y, dz = nodes[i].z - nodes[i - 1].z;
+        seg[i - 1] = std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double total = std::accumulate(seg.begin(), seg.end(), 0.0);
+    const unsigned R = nrPoints - (unsigned)M;
+    int cnt = 0, ns = 0;
+    auto push = [&](double x, double y, double z) {
+        if (out) {
+            out[3 * cnt] = x;
+            out[3 * cnt + 1] = y;
+            out[3 * cnt + 2] = z;
+        }
+        ++cnt;
+    };
+    auto mark = [&]() {
+        if (sympos) sympos[ns] = (unsigned)cnt;
+        ++ns;
+    };
+    if (total == 0.0) {
+        for (size_t j = 0; j < M; ++j) {
+            mark();
+            push(nodes[j].x, nodes[j].y, nodes[j].z);
+        }
+        for (unsigned i = 0; i < R; ++i) push(nodes.back().x, nodes.back().y, nodes.back().z);
+    } else {
+        std::vector<unsigned> m(M - 1);
+        std::vector<double> want(M - 1);
+        std::vector<size_t> order(M - 1);
+        for (size_t s = 0; s + 1 < M; ++s) {
+            want[s] = (seg[s] / total) * (double)R;
+            m[s] = (unsigned)std::floor(want[s]);
+            order[s] = s;
+        }
+        const unsigned rem = R - std::accumulate(m.begin(), m.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](size_t p, size_t q) {
+            return want[p] - std::floor(want[p]) > want[q] - std::floor(want[q]);
+        });
+        for (unsigned i = 0; i < rem; ++i) ++m[order[i]];
+        mark();
+        push(nodes[0].x, nodes[0].y, nodes[0].z);
+        for (size_t s = 0; s + 1 < M; ++s) {
+            const P3& A = nodes[s];
+            const double dx = nodes[s + 1].x - A.x, dy = nodes[s + 1].y - A.y, dz = nodes[s + 1].z - A.z;
+            const double denom = (double)(m[s] + 1);
+            for (unsigned i = 1; i <= m[s]; ++i) {
+                const double t = (double)i / denom;
+                push(A.x + dx * t, A.y + dy * t, A.z + dz * t);
+            }
+            mark();
+            push(nodes[s + 1].x, nodes[s + 1].y, nodes[s + 1].z);
+        }
+    }
+    if (count) *count = cnt;
+    return UEPM_OK;
+}
+
+int uepm_bz_sample(uint64_t seed, int64_t n, double* out) {
+    if (n < 0 || (n > 0 && !out)) return fail(UEPM_ERR_INVALID_ARG, "uepm_bz_sample: bad arguments");
+    uint64_t state = seed;
+    auto draw = [&]() {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    };
+    int64_t got = 0;
+    while (got < n) {
+        const double x = draw(), y = draw(), z = draw();
+        if (std::fabs(x) <= 1.0 && std::fabs(y) <= 1.0 && std::fabs(z) <= 1.0 &&
+            std::fabs(x) + std::fabs(y) + std::fabs(z) <= 3.0 / 2.0) {
+            out[3 * got] = x;
+            out[3 * got + 1] = y;
+            out[3 * got + 2] = z;
+            ++got;
+        }
+    }
+    return UEPM_OK;
+}
+
+int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, int N, unsigned flags, int nb,
+                     uepm_plan** out) {
+    if (!ctx || !m || !G || !out) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_create: null argument");
+    if (N < 1) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_create: empty basis");
+    const bool soc = flags & UEPM_SOC;
+    const int dim = soc ? 2 * N : N;
+    if (nb < 1 || nb > dim) return fail(UEPM_ERR_OUT_OF_RANGE, "uepm_plan_create: nb must be in [1, dim]");
+    if (!(m->lattice_constant_angstrom > 0.0)) return fail(UEPM_ERR_INVALID_ARG, "lattice constant must be positive");
+    if (soc) {
+        auto ok = [](int p) { return p == 2 || p == 3 || p == 4; };
+        if (!ok(m->period_cation))
+            return fail(UEPM_ERR_RUNTIME, "SpinOrbitCorrection::compute_lambda_1: Unsupported period for cation.");
+        if (!ok(m->period_anion))
+            return fail(UEPM_ERR_RUNTIME, "SpinOrbitCorrection::compute_lambda_2: Unsupported period for anion.");
+    }
+    if (flags & UEPM_GRADIENTS) flags |= UEPM_VECTORS;
+    HIP_OK(hipSetDevice(ctx->device));
+    auto* p = new uepm_plan;
+    p->ctx = ctx;
+    p->flags = flags;
+    p->N = N;
+    p->dim = dim;
+    p->nb = nb;
+    uepm::DevParams& P = p->P;
+    P.N = N;
+    P.dim = dim;
+    P.nl = (flags & UEPM_NONLOCAL) ? 1 : 0;
+    P.soc = soc ? 1 : 0;
+    P.nb = nb;
+    const double a = 1.0e-10 * m->lattice_constant_angstrom;  // Material.h:108-111
+    constexpr double one_eight = 1.0 / 8.0;
+    P.a = a;
+    P.tau = one_eight * a;
+    P.tpa = 2.0 * kPi / a;
+    const double diag_factor = (kHbar * kHbar) / (2.0 * kMe * kQe);  // Hamiltonian.cpp:68
+    P.kin = P.tpa * P.tpa * diag_factor;
+    for (int i = 0; i < 4; ++i) {
+        P.VS[i] = m->V_S[i];
+        P.VA[i] = m->V_A[i];
+    }
+    // Material.cpp:142-152, 170
+    P.fourier = 2.0 * kPi / a;
+    P.diag_factor = std::pow(kHbar, 2) / (2.0 * kMe * kQe);
+    const double kF = std::pow(96 * kPi * kPi, 1.0 / 3.0) / a;
+    P.kF2 = std::pow(kF, 2.0);
+    const double omega = (1.0 / 8.0) * std::pow(a, 3.0);
+    P.V_pre = 4.0 * kPi / omega;
+    P.tau_over_a = P.tau / a;
+    P.well = (m->nl_well_type == 0) ? 0 : 1;
+    P.alpha0_a = m->alpha_0_anion;
+    P.dfbeta_a = P.diag_factor * m->beta_0_anion;
+    P.A2_a = m->A2_anion;
+    P.R0_a = m->R0_anion;
+    P.R2_a = m->R2_anion;
+    P.gpre_a = 5.0 * std::pow(kPi, 1.5) * (std::pow(m->R2_anion, 3.0) / omega);
+    P.alpha0_c = m->alpha_0_cation;
+    P.dfbeta_c = P.diag_factor * m->beta_0_cation;
+    P.A2_c = m->A2_cation;
+    P.R0_c = m->R0_cation;
+    P.R2_c = m->R2_cation;
+    P.gpre_c = 5.0 * std::pow(kPi, 1.5) * (std::pow(m->R2_cation, 3.0) / omega);
+    // F_0 contributes only when R0 != 0 (Material.cpp:159,180) and its
+    // coefficient A_0 = alpha + diag*beta*(...) is not identically zero.
+    P.need_l0_a = (m->R0_anion != 0.0) && (m->alpha_0_anion != 0.0 || m->beta_0_anion != 0.0);
+    P.need_l0_c = (m->R0_cation != 0.0) && (m->alpha_0_cation != 0.0 || m->beta_0_cation != 0.0);
+    P.need_l2_a = m->A2_anion != 0.0;
+    P.need_l2_c = m->A2_cation != 0.0;
+    // SpinOrbitFunctional.cpp
+    P.soc_kfac = (2.0 * kPi) / a;
+    P.period_a = m->period_anion;
+    P.period_c = m->period_cation;
+    P.xi_a = m->radial_extent_anion;
+    P.xi_c = m->radial_extent_cation;
+    P.mu = m->mu_soc;
+    P.alpha_mu = m->alpha_soc * m->mu_soc;
+    P.bohr = kBohr;
+    // Hamiltonian.cpp:158-161
+    P.grad_pref = 2.0 * diag_factor * (P.tpa * P.tpa);
+    P.frac_to_phys = 1.0 / P.tpa;
+
+    hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
+    if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete p;
+        return fail(UEPM_ERR_DEVICE, std::string("uepm_plan_create: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return UEPM_OK;
+}
+
+int uepm_plan_destroy(uepm_plan* plan) {
+    delete plan;
+    return UEPM_OK;
+}
+
+int uepm_plan_dim(const uepm_plan* plan) { return plan ? plan->dim : -1; }
+
+int uepm_plan_set_timing(uepm_plan* plan, int enable) {
+    if (!plan) return fail(UEPM_ERR_INVALID_ARG, "null plan");
+    plan->timing = enable;
+    return UEPM_OK;
+}
+
+int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t* launches, const char** name) {
+    if (!plan) return fail(UEPM_ERR_INVALID_ARG, "null plan");
+    if (stage < 0 || stage >= kNumStages) return fail(UEPM_ERR_OUT_OF_RANGE, "no such stage");
+    if (ms) *ms = plan->stage_ms[stage];
+    if (launches) *launches = plan->stage_launches[stage];
+    if (name) *name = kStageNames[stage];
+    return UEPM_OK;
+}
+
+int uepm_assemble_device(uepm_plan* p, const double* dk, int64_t nk, double* dH, void* stream) {
+    if (!p || (nk > 0 && (!dk || !dH))) return fail(UEPM_ERR_INVALID_ARG, "uepm_assemble_device: null argument");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "nk < 0");
+    if (nk == 0) return UEPM_OK;
+    HIP_OK(hipSetDevice(p->ctx->device));
+    const int64_t chunk = 65535;
+    for (int64_t c0 = 0; c0 < nk; c0 += chunk) {
+        const int64_t c = std::min(chunk, nk - c0);
+        HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), dk + 3 * c0, c,
+                                     reinterpret_cast<double2*>(dH) + (size_t)c0 * p->dim * p->dim, p->dim,
+                                     (hipStream_t)stream));
+    }
+    return UEPM_OK;
+}
+
+int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals, double* devecs, double* dgrads,
+                      void* stream_) {
+    if (!p) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: null plan");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: nk < 0");
+    if (nk == 0) return UEPM_OK;
+    if (!dk || !devals) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: null k or eigenvalue buffer");
+    const bool vec = p->flags & UEPM_VECTORS;
+    if (devecs && !vec) return fail(UEPM_ERR_INVALID_ARG, "eigenvectors requested from a plan without UEPM_VECTORS");
+    if (dgrads && !(p->flags & UEPM_GRADIENTS))
+        return fail(UEPM_ERR_INVALID_ARG, "gradients requested from a plan without UEPM_GRADIENTS");
+    HIP_OK(hipSetDevice(p->ctx->device));
+    hipStream_t s = (hipStream_t)stream_;
+    const int n = p->dim, nb = p->nb;
+    const bool need_vec = vec && (devecs || dgrads);
+    // Batch size: the full-matrix workspace stays under ~4 GiB.
+    const size_t mat_bytes = sizeof(double2) * (size_t)n * n;
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)((4ull << 30) / mat_bytes)));
+    chunk = std::min(chunk, nk);
+    HIP_OK(p->H.ensure(mat_bytes * chunk));
+    HIP_OK(p->d.ensure(sizeof(double) * n * chunk));
+    HIP_OK(p->e.ensure(sizeof(double) * n * chunk));
+    HIP_OK(p->tau.ensure(sizeof(double2) * n * chunk));
+    if (need_vec) {
+        HIP_OK(p->Z.ensure(sizeof(double) * (size_t)n * nb * chunk));
+        HIP_OK(p->W.ensure(sizeof(double) * 6 * (size_t)n * nb * chunk));
+        if (!devecs) HIP_OK(p->X.ensure(sizeof(double2) * (size_t)n * nb * chunk));
+    }
+    const int nchunks = (int)((nk + chunk - 1) / chunk);
+    if (p->timing) {
+        const size_t need = (size_t)nchunks * kNumStages * 2;
+        while (p->ev.size() < need) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            p->ev.push_back(e);
+        }
+    }
+    auto rec = [&](int chunk_i, int stage, int end) -> hipError_t {
+        if (!p->timing) return hipSuccess;
+        return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end], s);
+    };
+    bool used[kNumStages] = {true, true, true, need_vec};
+    for (int ci = 0; ci < nchunks; ++ci) {
+        const int64_t c0 = (int64_t)ci * chunk;
+        const int64_t c = std::min(chunk, nk - c0);
+        const double* kc = dk + 3 * c0;
+        double2* Hc = p->H.as<double2>();
+        HIP_OK(rec(ci, kAssemble, 0));
+        HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, s));
+        HIP_OK(rec(ci, kAssemble, 1));
+        HIP_OK(rec(ci, kTridiag, 0));
+        HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
+        HIP_OK(rec(ci, kTridiag, 1));
+        HIP_OK(rec(ci, kTriEig, 0));
+        HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nb, c, devals + c0 * nb,
+                                    need_vec ? p->Z.as<double>() : nullptr, need_vec ? p->W.as<double>() : nullptr,
+                                    need_vec ? 1 : 0, s));
+        HIP_OK(rec(ci, kTriEig, 1));
+        if (need_vec) {
+            double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : p->X.as<double2>();
+            HIP_OK(rec(ci, kBack, 0));
+            HIP_OK(uepm::launch_backtransform(Hc, n, n, p->tau.as<double2>(), p->d.as<double>(), p->e.as<double>(),
+                                              devals + c0 * nb, p->Z.as<double>(), Xc,
+                                              dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), nb,
+                                              c, s));
+            HIP_OK(rec(ci, kBack, 1));
+        }
+    }
+    if (p->timing) {
+        HIP_OK(hipStreamSynchronize(s));
+        for (int st = 0; st < kNumStages; ++st) {
+            p->stage_ms[st] = 0.0;
+            p->stage_launches[st] = used[st] ? nchunks : 0;
+            if (!used[st]) continue;
+            for (int ci = 0; ci < nchunks; ++ci) {
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, p->ev[((size_t)ci * kNumStages + st) * 2],
+                                           p->ev[((size_t)ci * kNumStages + st) * 2 + 1]));
+                p->stage_ms[st] += ms;
+            }
+        }
+    }
+    return UEPM_OK;
+}
+
+int uepm_device_alloc(uepm_ctx* ctx, uint64_t bytes, void** ptr) {
+    if (!ctx || !ptr) return fail(UEPM_ERR_INVALID_ARG, "uepm_device_alloc: null argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    *ptr = nullptr;
+    if (bytes == 0) return UEPM_OK;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory) return fail(UEPM_ERR_NO_MEMORY, "uepm_device_alloc: out of device memory");
+    HIP_OK(e);
+    return UEPM_OK;
+}
+
+int uepm_device_free(uepm_ctx* ctx, void* ptr) {
+    if (!ctx) return fail(UEPM_ERR_INVALID_ARG, "uepm_device_free: null context");
+    if (!ptr) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipFree(ptr));
+    return UEPM_OK;
+}
+
+int uepm_memcpy_htod(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(UEPM_ERR_INVALID_ARG, "uepm_memcpy_htod: null argument");
+    if (!bytes) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return UEPM_OK;
+}
+
+int uepm_memcpy_dtoh(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(UEPM_ERR_INVALID_ARG, "uepm_memcpy_dtoh: null argument");
+    if (!bytes) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return UEPM_OK;
+}
+
+int uepm_memset_device(uepm_ctx* ctx, void* dst, int value, uint64_t bytes) {
+    if (!ctx || (bytes && !dst)) return fail(UEPM_ERR_INVALID_ARG, "uepm_memset_device: null argument");
+    if (!bytes) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipMemset(dst, value, bytes));
+    return UEPM_OK;
+}
+
+int uepm_stream_create(uepm_ctx* ctx, void** stream) {
+    if (!ctx || !stream) return fail(UEPM_ERR_INVALID_ARG, "uepm_stream_create: null argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t s;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return UEPM_OK;
+}
+
+int uepm_stream_destroy(uepm_ctx* ctx, void* stream) {
+    if (!ctx) return fail(UEPM_ERR_INVALID_ARG, "uepm_stream_destroy: null context");
+    if (!stream) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipStreamDestroy((hipStream_t)stream));
+    return UEPM_OK;
+}
+
+int uepm_stream_synchronize(uepm_ctx* ctx, void* stream) {
+    if (!ctx) return fail(UEPM_ERR_INVALID_ARG, "uepm_stream_synchronize: null context");
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipStreamSynchronize((hipStream_t)stream));
+    return UEPM_OK;
+}
+
+int uepm_device_synchronize(uepm_ctx* ctx) {
+    if (!ctx) return fail(UEPM_ERR_INVALID_ARG, "uepm_device_synchronize: null context");
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipDeviceSynchronize());
+    return UEPM_OK;
+}
+
+int uepm_solve(uepm_plan* p, const double* k, int64_t nk, double* evals, double* evecs, double* grads) {
+    if (!p) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: null plan");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: nk < 0");
+    if (nk == 0) return UEPM_OK;
+    if (!k || !evals) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: null k or eigenvalue buffer");
+    HIP_OK(hipSetDevice(p->ctx->device));
+    const int n = p->dim, nb = p->nb;
+    DeviceBuffer dk, dev, dvec, dgr;
+    HIP_OK(dk.ensure(sizeof(double) * 3 * nk));
+    HIP_OK(dev.ensure(sizeof(double) * nb * nk));
+    if (evecs) HIP_OK(dvec.ensure(sizeof(double2) * (size_t)n * nb * nk));
+    if (grads) HIP_OK(dgr.ensure(sizeof(double) * 3 * nb * nk));
+    HIP_OK(hipMemcpy(dk.p, k, sizeof(double) * 3 * nk, hipMemcpyHostToDevice));
+    int r = uepm_solve_device(p, dk.as<double>(), nk, dev.as<double>(), evecs ? dvec.as<double>() : nullptr,
+                              grads ? dgr.as<double>() : nullptr, nullptr);
+    if (r != UEPM_OK) return r;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMemcpy(evals, dev.p, sizeof(double) * nb * nk, hipMemcpyDeviceToHost));
+    if (evecs) HIP_OK(hipMemcpy(evecs, dvec.p, sizeof(double2) * (size_t)n * nb * nk, hipMemcpyDeviceToHost));
+    if (grads) HIP_OK(hipMemcpy(grads, dgr.p, sizeof(double) * 3 * nb * nk, hipMemcpyDeviceToHost));
+    return UEPM_OK;
+}
+
+}  // extern "C"
