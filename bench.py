@@ -1,0 +1,173 @@
+"""Benchmark: k-points/s of the batched EPM band solve on one MI355X.
+
+Workload (one "step"): Si with non-local corrections (materials.yaml), basis
+n = 12 nearest shells (N = 181, complex128 Hermitian H(k)), 200,000 k-points
+drawn with the seeded first-BZ sampler (seed 20261202), lowest 12 eigenvalues
+plus eigenvectors, one batched solve call with inputs resident in HBM.
+
+Prints one JSON line (see the metric / roofline / cpu_baseline fields).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "k-pts/s Si nonlocal N≈181 FP64 vs mixed @≤0.1 meV; serving p50/p99 latency"
+FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+SEED = 20261202
+
+
+def tridiag_flops(n: int) -> float:
+    """Algorithmic real flops of the Hermitian Householder reduction: (16/3) n^3."""
+    return 16.0 / 3.0 * n ** 3
+
+
+def cpu_baseline(mat, G, nb, ks, seconds: float):
+    """The oracle (C++ restatement of Compute_parallel, OpenMP) on a bounded sample."""
+    from oracle import oracle as O
+    from ultimateepm_amd import api
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    flags = api.NONLOCAL | api.VECTORS
+    probe = ks[: 2 * threads]
+    t0 = time.perf_counter()
+    O.solve(mat, G, flags, nb, probe, nthreads=threads)
+    rate = len(probe) / (time.perf_counter() - t0)
+    m = int(max(2 * threads, min(len(ks), rate * seconds)))
+    sample = ks[:m]
+    t0 = time.perf_counter()
+    O.solve(mat, G, flags, nb, sample, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(m / dt, 2),
+        "unit": "k-pts/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"first {m} k-points of the same seeded BZ set, Si non-local N={len(G)}, {nb} bands + "
+                  f"eigenvectors (all eigenvectors, as Eigen computes them), oracle built -O3 "
+                  f"-march=x86-64-v3 -ffp-contract=off, {threads} OpenMP threads, {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nk", type=int, default=200_000)
+    ap.add_argument("--nb", type=int, default=12)
+    ap.add_argument("--shells", type=int, default=12)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the k-point batch shards across GPUs by "
+                                   "block partition (see DESIGN.md), but no multi-GPU run is part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    from ultimateepm_amd import api
+    from ultimateepm_amd.material import load
+
+    mat = load("Si")
+    G = api.basis_vectors(args.shells)
+    N = len(G)
+    nb = args.nb
+    ctx = api.Context(0)
+    flags = api.NONLOCAL | api.VECTORS
+    plan = api.Plan(ctx, mat, G, flags, nb)
+    ks = api.bz_sample(SEED, args.nk)
+    dk = api.DeviceArray.from_host(ctx, ks)
+    dev = api.DeviceArray(ctx, (args.nk, nb))
+    dvec = api.DeviceArray(ctx, (args.nk, nb, plan.dim, 2))
+    stream = ctx.stream()
+
+    def step():
+        plan.solve_device(dk.ptr, args.nk, dev.ptr, dvec.ptr, 0, stream.handle)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    plan.set_timing(True)
+    stage_ms = {}
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        for name, (ms, n) in plan.last_timing().items():
+            acc = stage_ms.setdefault(name, [0.0, 0])
+            acc[0] += ms
+            acc[1] += n
+    stream.synchronize()
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = args.nk / (elapsed / args.steps)
+
+    dominant = max(stage_ms, key=lambda k: stage_ms[k][0])
+    dom_ms, dom_launches = stage_ms[dominant]
+    flops = tridiag_flops(plan.dim) * args.nk * args.steps
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    roofline = {
+        "kernel": dominant,
+        "bound": "mfma",
+        "achieved": round(achieved, 3),
+        "peak": FP64_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
+        "flop_per_k": tridiag_flops(plan.dim),
+        "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stage_ms.items()},
+    }
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "k-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Si non-local (materials.yaml), n={args.shells} shells (N={N}), {args.nk} seeded "
+                        f"first-BZ k-points, lowest {nb} eigenvalues + eigenvectors, complex128",
+            "mode": "fp64",
+            "nk": args.nk,
+            "N": N,
+            "nb": nb,
+        },
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    plan.close()
+    stream.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
