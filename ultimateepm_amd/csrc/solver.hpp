@@ -9,6 +9,21 @@
 
 namespace uepm {
 
+// Layout of the blocked reduction for one matrix size: leading columns [0, c0)
+// in global scratch, trailing columns in LDS (lds_bytes dynamic LDS per workgroup).
+struct TriPlan {
+    int n;
+    int bs;  // threads per workgroup (256 or 512)
+    int c0;
+    int lds_bytes;
+    int64_t scratch_elems;  // complex elements of global scratch per workgroup
+};
+TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs);
+hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
+                                      double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s);
+hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
+                                      float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s);
+
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s);
 hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, double* de, double2* dtau,

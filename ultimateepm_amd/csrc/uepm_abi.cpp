@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -72,6 +73,7 @@ const char* kStageNames[kNumStages] = {"assemble_kernel", "tridiag_kernel", "tri
 struct uepm_ctx {
     int device = 0;
     hipDeviceProp_t prop{};
+    int lds_limit = 65536;  // dynamic LDS bytes one workgroup may use
 };
 
 struct uepm_plan {
@@ -79,7 +81,9 @@ struct uepm_plan {
     uepm::DevParams P{};
     unsigned flags = 0;
     int N = 0, dim = 0, nb = 0;
-    DeviceBuffer G, H, d, e, tau, Z, W, X;
+    DeviceBuffer G, H, d, e, tau, Z, W, X, S;
+    uepm::TriPlan tp{};
+    bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
     double stage_ms[kNumStages] = {0, 0, 0, 0};
@@ -105,6 +109,9 @@ int uepm_ctx_create(int device, uepm_ctx** out) {
         delete c;
         return fail(UEPM_ERR_DEVICE, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        c->lds_limit = lds;
     *out = c;
     return UEPM_OK;
 }
@@ -330,6 +337,11 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     P.grad_pref = 2.0 * diag_factor * (P.tpa * P.tpa);
     P.frac_to_phys = 1.0 / P.tpa;
 
+    const char* bs_env = std::getenv("UEPM_TRI_BS");
+    const int tri_bs = (bs_env && std::atoi(bs_env) == 256) ? 256 : 512;
+    p->tp = uepm::tri_plan(dim, (int)sizeof(double2), ctx->lds_limit, tri_bs);
+    const char* v1 = std::getenv("UEPM_TRIDIAG_V1");
+    p->tri_v1 = v1 && v1[0] == '1';
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
     if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -399,6 +411,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     HIP_OK(p->d.ensure(sizeof(double) * n * chunk));
     HIP_OK(p->e.ensure(sizeof(double) * n * chunk));
     HIP_OK(p->tau.ensure(sizeof(double2) * n * chunk));
+    if (!p->tri_v1) HIP_OK(p->S.ensure(sizeof(double2) * (size_t)p->tp.scratch_elems * chunk));
     if (need_vec) {
         HIP_OK(p->Z.ensure(sizeof(double) * (size_t)n * nb * chunk));
         HIP_OK(p->W.ensure(sizeof(double) * 6 * (size_t)n * nb * chunk));
@@ -427,7 +440,11 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, s));
         HIP_OK(rec(ci, kAssemble, 1));
         HIP_OK(rec(ci, kTridiag, 0));
-        HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
+        if (p->tri_v1)
+            HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
+        else
+            HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, p->S.p, Hc, p->d.as<double>(), p->e.as<double>(),
+                                                    p->tau.as<double2>(), s));
         HIP_OK(rec(ci, kTridiag, 1));
         HIP_OK(rec(ci, kTriEig, 0));
         HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nb, c, devals + c0 * nb,
