@@ -74,6 +74,7 @@ def main() -> int:
     ap.add_argument("--shells", type=int, default=12)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -90,52 +91,77 @@ def main() -> int:
     N = len(G)
     nb = args.nb
     ctx = api.Context(0)
-    flags = api.NONLOCAL | api.VECTORS
-    plan = api.Plan(ctx, mat, G, flags, nb)
     ks = api.bz_sample(SEED, args.nk)
     dk = api.DeviceArray.from_host(ctx, ks)
-    dev = api.DeviceArray(ctx, (args.nk, nb))
-    dvec = api.DeviceArray(ctx, (args.nk, nb, plan.dim, 2))
     stream = ctx.stream()
 
-    def step():
-        plan.solve_device(dk.ptr, args.nk, dev.ptr, dvec.ptr, 0, stream.handle)
+    def run(mode: str):
+        flags = api.NONLOCAL | api.VECTORS | (api.MIXED if mode == "mixed" else 0)
+        plan = api.Plan(ctx, mat, G, flags, nb)
+        dev = api.DeviceArray(ctx, (args.nk, nb))
+        dvec = api.DeviceArray(ctx, (args.nk, nb, plan.dim, 2))
 
-    for _ in range(args.warmup):
-        step()
-    stream.synchronize()
-    plan.set_timing(True)
-    stage_ms = {}
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        for name, (ms, n) in plan.last_timing().items():
-            acc = stage_ms.setdefault(name, [0.0, 0])
-            acc[0] += ms
-            acc[1] += n
-    stream.synchronize()
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    value = args.nk / (elapsed / args.steps)
+        def step():
+            plan.solve_device(dk.ptr, args.nk, dev.ptr, dvec.ptr, 0, stream.handle)
 
+        for _ in range(args.warmup):
+            step()
+        stream.synchronize()
+        plan.set_timing(True)
+        stage_ms = {}
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+            for name, (ms, cnt) in plan.last_timing().items():
+                acc = stage_ms.setdefault(name, [0.0, 0])
+                acc[0] += ms
+                acc[1] += cnt
+        stream.synchronize()
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        evals = dev.numpy()
+        plan.close()
+        dev.free()
+        dvec.free()
+        return {"elapsed": elapsed, "stage_ms": stage_ms, "evals": evals, "dim": plan.dim}
+
+    res = {m: run(m) for m in args.modes.split(",")}
+    head = "mixed" if "mixed" in res else "fp64"
+    r = res[head]
+    ms_per_step = r["elapsed"] / args.steps * 1e3
+    value = args.nk / (r["elapsed"] / args.steps)
+    stage_ms = r["stage_ms"]
     dominant = max(stage_ms, key=lambda k: stage_ms[k][0])
     dom_ms, dom_launches = stage_ms[dominant]
-    flops = tridiag_flops(plan.dim) * args.nk * args.steps
+    if dominant == "tridiag_kernel":
+        flops = tridiag_flops(r["dim"]) * args.nk * args.steps
+        peak = FP32_MFMA_PEAK_TFLOPS if head == "mixed" else FP64_PEAK_TFLOPS
+    else:
+        flops = 0.0
+        peak = FP64_PEAK_TFLOPS
     achieved = flops / (dom_ms * 1e-3) / 1e12
     roofline = {
         "kernel": dominant,
         "bound": "mfma",
         "achieved": round(achieved, 3),
-        "peak": FP64_PEAK_TFLOPS,
+        "peak": peak,
         "unit": "TFLOP/s",
-        "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+        "frac": round(achieved / peak, 4),
         "traffic": None,
         "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
-        "flop_per_k": tridiag_flops(plan.dim),
+        "flop_per_k": tridiag_flops(r["dim"]),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stage_ms.items()},
     }
+    extra = {}
+    for m, rr in res.items():
+        extra[f"{m}_kpts_per_s"] = round(args.nk / (rr["elapsed"] / args.steps), 1)
+        extra[f"{m}_stage_ms_per_step"] = {k: round(v[0] / args.steps, 3) for k, v in rr["stage_ms"].items()}
+    if "mixed" in res and "fp64" in res:
+        d = res["mixed"]["evals"] - res["fp64"]["evals"]
+        extra["mixed_vs_fp64_max_meV"] = float(np.abs(d).max() * 1e3)
+        extra["mixed_vs_fp64_rms_meV"] = float(np.sqrt(np.mean(d ** 2)) * 1e3)
+        extra["mixed_over_fp64"] = round(extra["mixed_kpts_per_s"] / extra["fp64_kpts_per_s"], 3)
 
     out = {
         "metric": METRIC,
@@ -148,22 +174,22 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "c64+f64" if head == "mixed" else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"Si non-local (materials.yaml), n={args.shells} shells (N={N}), {args.nk} seeded "
-                        f"first-BZ k-points, lowest {nb} eigenvalues + eigenvectors, complex128",
-            "mode": "fp64",
+                        f"first-BZ k-points, lowest {nb} eigenvalues + eigenvectors",
+            "mode": head,
             "nk": args.nk,
             "N": N,
             "nb": nb,
+            **extra,
         },
         "roofline": roofline,
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)
-    plan.close()
     stream.close()
     ctx.close()
     return 0

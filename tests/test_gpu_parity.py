@@ -159,3 +159,36 @@ def test_soc_unsupported_period(uepm, gpu_ctx):
     with pytest.raises(UepmError) as ex:
         uepm.Plan(gpu_ctx, mat, G, uepm.SOC, 4)
     assert ex.value.code == UEPM_ERR_RUNTIME
+
+
+MIXED_TOL = 1e-4  # eV (0.1 meV): precision form vs the FP64 oracle
+
+
+@pytest.mark.parametrize("sym,n,soc,nb", [
+    ("Si", 12, False, 12),   # the headline shape: N = 181
+    ("Ge", 10, True, 16),    # 2N = 274, Kramers pairs
+    ("GaAs", 8, True, 12),   # 2N = 178, zincblende with SOC
+])
+def test_mixed_precision_vs_oracle(uepm, oracle, gpu_ctx, sym, n, soc, nb):
+    mat = _mat(sym)
+    G = uepm.basis_vectors(n)
+    flags = uepm.NONLOCAL | (uepm.SOC if soc else 0)
+    plan = uepm.Plan(gpu_ctx, mat, G, flags | uepm.MIXED | uepm.VECTORS, nb)
+    ks = np.vstack([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5], uepm.bz_sample(99, 30)])
+    ev, vec, _ = plan.solve(ks)
+    w_o, _, _ = oracle.solve(mat, G, flags, nb, ks, nthreads=8)
+    err = ev - w_o
+    print(f"{sym}: max |dE| = {np.abs(err).max() * 1e3:.3e} meV, rms = {np.sqrt(np.mean(err ** 2)) * 1e3:.3e} meV")
+    assert np.abs(err).max() <= MIXED_TOL
+    for i, k in enumerate(ks[:6]):
+        H = oracle.assemble(mat, G, k, nl=True, soc=soc)
+        X = vec[i].T
+        orth = np.abs(X.conj().T @ X - np.eye(nb)).max()
+        res = np.linalg.norm(H @ X - X * ev[i], axis=0).max()
+        assert orth <= 1e-10, orth
+        assert res <= 1e-3, res  # vectors carry the complex64 reduction's accuracy
+    if soc and sym in ("Si", "Ge"):
+        # diamond structure: inversion + time reversal make every band a Kramers pair
+        # (zincblende GaAs lacks inversion, so its SOC bands are spin-split)
+        pairs = ev.reshape(len(ks), -1, 2)
+        assert np.abs(pairs[:, :, 0] - pairs[:, :, 1]).max() <= MIXED_TOL

@@ -18,7 +18,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip"]
+DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip"]
 HOST_SOURCES = ["uepm_abi.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,7 +39,8 @@ def build(verbose: bool = True) -> str:
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD_DIR, src + ".o")
-        _run([HIPCC, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", *COMMON, "-c",
+              os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])

@@ -35,4 +35,15 @@ hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2*
                                 const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
                                 hipStream_t s);
 
+hipError_t launch_backtransform_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dd,
+                                    const double* de, const double* devals, const double* dZ, double2* dX,
+                                    const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
+                                    hipStream_t s);
+// FP64 Rayleigh-Ritz refinement of nbr approximate eigenvectors X (nk x nbr x n,
+// overwritten) against the FP64 Hamiltonians H (nk x n x n column-major):
+// writes the lowest nb refined eigenvalues / vectors / gradients.
+hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr, int nb, double* devals,
+                         double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
+                         int64_t nk, hipStream_t s, double2* dY);
+
 }  // namespace uepm

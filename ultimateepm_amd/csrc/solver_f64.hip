@@ -382,9 +382,14 @@ __global__ void tri_eig_kernel(const double* __restrict__ dall, const double* __
 // Z: nk x nb x n real eigenvectors of T (unit norm).  X (output, complex):
 // nk x nb x dim.  Bands within ortol of each other are re-orthogonalised
 // (modified Gram-Schmidt, ascending) before x = Q z.
-template <int BS>
-__global__ __launch_bounds__(BS) void backtransform_kernel(const double2* __restrict__ H, int n, int ld,
-                                                           const double2* __restrict__ tauall,
+__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
+__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
+
+// RT: reflector storage type (double2 for the FP64 path, float2 for the
+// complex64 reduction of the mixed-precision path); arithmetic is FP64.
+template <int BS, typename RT>
+__global__ __launch_bounds__(BS) void backtransform_kernel(const RT* __restrict__ H, int n, int ld,
+                                                           const RT* __restrict__ tauall,
                                                            const double* __restrict__ dall,
                                                            const double* __restrict__ eall,
                                                            const double* __restrict__ evals, const double* Z,
@@ -394,8 +399,8 @@ __global__ __launch_bounds__(BS) void backtransform_kernel(const double2* __rest
     const int kk = blockIdx.x;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     constexpr int NW = BS / 64;
-    const double2* A = H + (size_t)kk * ld * n;
-    const double2* tau = tauall + (size_t)kk * n;
+    const RT* A = H + (size_t)kk * ld * n;
+    const RT* tau = tauall + (size_t)kk * n;
     const double* z = Z + (size_t)kk * nb * n;
     double2* x = X + (size_t)kk * nb * n;
     const double* ev = evals + (size_t)kk * nb;
@@ -438,13 +443,13 @@ __global__ __launch_bounds__(BS) void backtransform_kernel(const double2* __rest
     for (int b = wv; b < nb; b += NW) {
         double2* xb = x + (size_t)b * n;
         for (int j = n - 2; j >= 0; --j) {
-            const double2 tj = tau[j];
+            const double2 tj = to_d2(tau[j]);
             if (tj.x == 0.0 && tj.y == 0.0) continue;
-            const double2* vj = A + (size_t)j * ld;  // v at rows j+1..n-1, v[j+1] = 1
+            const RT* vj = A + (size_t)j * ld;  // v at rows j+1..n-1, v[j+1] = 1
             const int r0 = lane + (j + 1 > lane ? ((j + 1 - lane + 63) >> 6) << 6 : 0);
             double sr = 0.0, si = 0.0;
             for (int r = r0; r < n; r += 64) {
-                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : vj[r];
+                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : to_d2(vj[r]);
                 const double2 q = cmulc(vr, xb[r]);
                 sr += q.x;
                 si += q.y;
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(BS) void backtransform_kernel(const double2* __rest
             si = wave_sum(si);
             const double2 s = cmul(tj, make_double2(sr, si));
             for (int r = r0; r < n; r += 64) {
-                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : vj[r];
+                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : to_d2(vj[r]);
                 xb[r] = csub(xb[r], cmul(s, vr));
             }
         }
@@ -515,8 +520,17 @@ hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2*
                                 const double* de, const double* devals, const double* dZ, double2* dX, double* dgrads,
                                 const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
                                 hipStream_t s) {
-    hipLaunchKernelGGL(backtransform_kernel<kBtBS>, dim3((unsigned)nk), dim3(kBtBS), 0, s, dH, n, ld, dtau, dd, de,
-                       devals, dZ, dX, dgrads, P, dk, dG, nb);
+    hipLaunchKernelGGL((backtransform_kernel<kBtBS, double2>), dim3((unsigned)nk), dim3(kBtBS), 0, s, dH, n, ld, dtau,
+                       dd, de, devals, dZ, dX, dgrads, P, dk, dG, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_backtransform_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dd,
+                                    const double* de, const double* devals, const double* dZ, double2* dX,
+                                    const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL((backtransform_kernel<kBtBS, float2>), dim3((unsigned)nk), dim3(kBtBS), 0, s, dV, n, ld, dtau,
+                       dd, de, devals, dZ, dX, nullptr, P, dk, dG, nb);
     return hipGetLastError();
 }
 

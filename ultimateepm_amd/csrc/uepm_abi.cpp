@@ -65,8 +65,12 @@ struct DeviceBuffer {
     }
 };
 
-enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kNumStages };
-const char* kStageNames[kNumStages] = {"assemble_kernel", "tridiag_kernel", "tri_eig_kernel", "backtransform_kernel"};
+enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kRefine, kNumStages };
+const char* kStageNames[kNumStages] = {"assemble_kernel", "tridiag_kernel", "tri_eig_kernel", "backtransform_kernel",
+                                       "refine_kernel"};
+// Extra Ritz vectors carried by the precision form beyond the nb requested,
+// so that a degenerate pair straddling band nb is refined as a whole.
+constexpr int kMixedExtra = 4;
 
 }  // namespace
 
@@ -82,12 +86,13 @@ struct uepm_plan {
     unsigned flags = 0;
     int N = 0, dim = 0, nb = 0;
     DeviceBuffer G, H, d, e, tau, Z, W, X, S;
-    uepm::TriPlan tp{};
+    DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
+    uepm::TriPlan tp{}, tp32{};
     bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
-    double stage_ms[kNumStages] = {0, 0, 0, 0};
-    int64_t stage_launches[kNumStages] = {0, 0, 0, 0};
+    double stage_ms[kNumStages] = {0, 0, 0, 0, 0};
+    int64_t stage_launches[kNumStages] = {0, 0, 0, 0, 0};
     ~uepm_plan() {
         for (auto e : ev) (void)hipEventDestroy(e);
     }
@@ -273,6 +278,8 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
             return fail(UEPM_ERR_RUNTIME, "SpinOrbitCorrection::compute_lambda_2: Unsupported period for anion.");
     }
     if (flags & UEPM_GRADIENTS) flags |= UEPM_VECTORS;
+    if ((flags & UEPM_MIXED) && std::min(dim, nb + kMixedExtra) > 28)
+        return fail(UEPM_ERR_OUT_OF_RANGE, "uepm_plan_create: the precision form refines at most 24 bands");
     HIP_OK(hipSetDevice(ctx->device));
     auto* p = new uepm_plan;
     p->ctx = ctx;
@@ -340,6 +347,7 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     const char* bs_env = std::getenv("UEPM_TRI_BS");
     const int tri_bs = (bs_env && std::atoi(bs_env) == 256) ? 256 : 512;
     p->tp = uepm::tri_plan(dim, (int)sizeof(double2), ctx->lds_limit, tri_bs);
+    p->tp32 = uepm::tri_plan(dim, (int)sizeof(float2), ctx->lds_limit, tri_bs);
     const char* v1 = std::getenv("UEPM_TRIDIAG_V1");
     p->tri_v1 = v1 && v1[0] == '1';
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
@@ -403,6 +411,9 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     hipStream_t s = (hipStream_t)stream_;
     const int n = p->dim, nb = p->nb;
     const bool need_vec = vec && (devecs || dgrads);
+    const bool mixed = p->flags & UEPM_MIXED;
+    const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
+    const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
     // Batch size: the full-matrix workspace stays under ~4 GiB.
     const size_t mat_bytes = sizeof(double2) * (size_t)n * n;
     int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)((4ull << 30) / mat_bytes)));
@@ -411,11 +422,20 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     HIP_OK(p->d.ensure(sizeof(double) * n * chunk));
     HIP_OK(p->e.ensure(sizeof(double) * n * chunk));
     HIP_OK(p->tau.ensure(sizeof(double2) * n * chunk));
-    if (!p->tri_v1) HIP_OK(p->S.ensure(sizeof(double2) * (size_t)p->tp.scratch_elems * chunk));
-    if (need_vec) {
-        HIP_OK(p->Z.ensure(sizeof(double) * (size_t)n * nb * chunk));
-        HIP_OK(p->W.ensure(sizeof(double) * 6 * (size_t)n * nb * chunk));
-        if (!devecs) HIP_OK(p->X.ensure(sizeof(double2) * (size_t)n * nb * chunk));
+    if (mixed) {
+        HIP_OK(p->Vf.ensure(sizeof(float2) * (size_t)n * n * chunk));
+        HIP_OK(p->tauf.ensure(sizeof(float2) * (size_t)n * chunk));
+        HIP_OK(p->S32.ensure(sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk));
+        HIP_OK(p->Ev.ensure(sizeof(double) * (size_t)nbr * chunk));
+        HIP_OK(p->X0.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
+        HIP_OK(p->Y.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
+    } else if (!p->tri_v1) {
+        HIP_OK(p->S.ensure(sizeof(double2) * (size_t)p->tp.scratch_elems * chunk));
+    }
+    if (stage_vec) {
+        HIP_OK(p->Z.ensure(sizeof(double) * (size_t)n * nbr * chunk));
+        HIP_OK(p->W.ensure(sizeof(double) * 6 * (size_t)n * nbr * chunk));
+        if (!mixed && !devecs) HIP_OK(p->X.ensure(sizeof(double2) * (size_t)n * nb * chunk));
     }
     const int nchunks = (int)((nk + chunk - 1) / chunk);
     if (p->timing) {
@@ -430,7 +450,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         if (!p->timing) return hipSuccess;
         return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end], s);
     };
-    bool used[kNumStages] = {true, true, true, need_vec};
+    bool used[kNumStages] = {true, true, true, stage_vec, mixed};
     for (int ci = 0; ci < nchunks; ++ci) {
         const int64_t c0 = (int64_t)ci * chunk;
         const int64_t c = std::min(chunk, nk - c0);
@@ -439,6 +459,31 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         HIP_OK(rec(ci, kAssemble, 0));
         HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, s));
         HIP_OK(rec(ci, kAssemble, 1));
+        if (mixed) {
+            // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
+            // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
+            float2* Vc = p->Vf.as<float2>();
+            HIP_OK(rec(ci, kTridiag, 0));
+            HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
+                                                    p->e.as<double>(), p->tauf.as<float2>(), s));
+            HIP_OK(rec(ci, kTridiag, 1));
+            HIP_OK(rec(ci, kTriEig, 0));
+            HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nbr, c, p->Ev.as<double>(),
+                                        p->Z.as<double>(), p->W.as<double>(), 1, s));
+            HIP_OK(rec(ci, kTriEig, 1));
+            HIP_OK(rec(ci, kBack, 0));
+            HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, p->tauf.as<float2>(), p->d.as<double>(), p->e.as<double>(),
+                                                  p->Ev.as<double>(), p->Z.as<double>(), p->X0.as<double2>(), p->P, kc,
+                                                  p->G.as<int32_t>(), nbr, c, s));
+            HIP_OK(rec(ci, kBack, 1));
+            HIP_OK(rec(ci, kRefine, 0));
+            HIP_OK(uepm::launch_refine(Hc, n, n, p->X0.as<double2>(), nbr, nb, devals + c0 * nb,
+                                       devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
+                                       dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, s,
+                                       p->Y.as<double2>()));
+            HIP_OK(rec(ci, kRefine, 1));
+            continue;
+        }
         HIP_OK(rec(ci, kTridiag, 0));
         if (p->tri_v1)
             HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
