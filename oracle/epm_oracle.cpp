@@ -29,6 +29,26 @@
  * the implicit-shift QL iteration (Wilkinson-Reinsch tql2, Handbook for
  * Automatic Computation vol. II, 1971), the same backward-stable algorithm
  * family; eigenvalues agree with Eigen to ~c*N*eps*||H||.
+ *
+ * How each part is pinned (tests/test_oracle_pins.py, fixtures in tests/golden/):
+ *   Bessel j_l, J_nu      the reference's known-answer tables
+ *                         (tests/epp/test_bessel_functions.cpp:22-65), bit-equal to
+ *                         bessel_func.hpp compiled in oracle/_ref, scipy spherical_jn
+ *   i_2                   scipy spherical_in(2, x); bit-equal to oracle/_ref
+ *   F_l, gaussian F_2     numerical quadrature of the square / gaussian well overlap
+ *   local V               bit-equal to Pseudopotential::GetValue compiled in oracle/_ref
+ *   k path                bit-equal to SymmetryPoints::GeneratePoints in oracle/_ref
+ *   basis                 closed-form N table and brute-force enumeration
+ *   kinetic + gradient    free-electron limit (closed form), finite differences
+ *   spin-orbit            radial factors by quadrature of Slater np orbitals, Pauli
+ *                         form of the 2x2 block, Kramers pairs, mu=0 doubling
+ *   eigensolver           LAPACK zheevd goldens (1e-10 eV), random Hermitian
+ *                         matrices, O_h invariance of the bands
+ * Parity unpinned: the bit-level output of Eigen 5.0.0's SelfAdjointEigenSolver
+ * (eigenvector phases, order inside degenerate clusters, rounding below ~1e-11 eV).
+ * Eigen is absent from this image and no reference test pins it; eigenvalues are
+ * pinned to LAPACK instead and eigenvectors are compared by residual and by
+ * cluster projectors.
  */
 #include <algorithm>
 #include <cmath>
@@ -533,6 +553,36 @@ int orc_kpath(const char* labels, unsigned nrPoints, double* out, unsigned* symp
 double orc_generalized_bessel(double nu, double x) { return gen_bessel(nu, x); }
 double orc_generalized_bessel_cylindrical(double nu, double x) { return gen_bessel_cyl(nu, x); }
 double orc_i2(double x) { return i2(x); }
+
+/* Radial overlap F_l (Material.cpp:87-109) for physical K1, K2 [1/m] and R [m]. */
+double orc_F_l(const double* K1, const double* K2, double R, int l) {
+    return F_l({K1[0], K1[1], K1[2]}, {K2[0], K2[1], K2[2]}, R, l);
+}
+/* Gaussian-well l=2 radial factor (Material.cpp:111-116). */
+double orc_F2_gauss(const double* K1, const double* K2, double R) {
+    return F_2_gauss({K1[0], K1[1], K1[2]}, {K2[0], K2[1], K2[2]}, R);
+}
+/* Spin-orbit radial factor B_period(|K|) (SpinOrbitFunctional.cpp:20-49); -1e300 for an unsupported period. */
+double orc_B_radial(int period, double Klen, double xi) {
+    try {
+        return B_radial(period, Klen, xi);
+    } catch (const std::exception&) {
+        return -1e300;
+    }
+}
+/* One 2x2 spin-orbit block for reduced K = k+G, Kp = k+G' (SpinOrbitFunctional.cpp:100-138);
+ * out = [s00, s01, s10, s11] as (re, im) pairs. */
+void orc_soc_block(const uepm_material* m, const double* K, const double* Kp, const double* G, const double* Gp,
+                   double* out) {
+    const double a = lattice_m(*m);
+    const V3 tau = {a / 8.0, a / 8.0, a / 8.0};
+    cplx s[4];
+    soc_block(*m, {K[0], K[1], K[2]}, {Kp[0], Kp[1], Kp[2]}, {G[0], G[1], G[2]}, {Gp[0], Gp[1], Gp[2]}, tau, s);
+    for (int q = 0; q < 4; ++q) {
+        out[2 * q] = s[q].real();
+        out[2 * q + 1] = s[q].imag();
+    }
+}
 
 /* Single non-local element for reduced K1 = k+G_i, K2 = k+G_j; out = (re, im). */
 void orc_nl_element(const uepm_material* m, const double* K1n, const double* K2n, double* out) {

@@ -57,6 +57,13 @@ def lib() -> C.CDLL:
         L.orc_adjust_values.restype = C.c_double
         L.orc_adjust_values.argtypes = [D, C.c_int64, C.c_int, C.c_int, C.c_int]
         L.orc_ryd_to_ev.restype = C.c_double
+        L.orc_F_l.restype = C.c_double
+        L.orc_F_l.argtypes = [D, D, C.c_double, C.c_int]
+        L.orc_F2_gauss.restype = C.c_double
+        L.orc_F2_gauss.argtypes = [D, D, C.c_double]
+        L.orc_B_radial.restype = C.c_double
+        L.orc_B_radial.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_soc_block.argtypes = [P, D, D, D, D, D]
         _lib = L
     return _lib
 
@@ -191,3 +198,31 @@ def splitmix64_bz_sample(seed: int, n: int) -> np.ndarray:
         if abs(v[0]) <= 1 and abs(v[1]) <= 1 and abs(v[2]) <= 1 and abs(v[0]) + abs(v[1]) + abs(v[2]) <= 1.5:
             out.append(v)
     return np.array(out, dtype=np.float64).reshape(-1, 3)
+
+
+def F_l(K1, K2, R: float, l: int) -> float:
+    """Radial overlap F_l for physical wave vectors K1, K2 [1/m] and well radius R [m]."""
+    return lib().orc_F_l(_d(np.ascontiguousarray(K1, dtype=np.float64)),
+                         _d(np.ascontiguousarray(K2, dtype=np.float64)), R, l)
+
+
+def F2_gauss(K1, K2, R: float) -> float:
+    return lib().orc_F2_gauss(_d(np.ascontiguousarray(K1, dtype=np.float64)),
+                              _d(np.ascontiguousarray(K2, dtype=np.float64)), R)
+
+
+def B_radial(period: int, K: float, xi: float) -> float:
+    v = lib().orc_B_radial(period, K, xi)
+    if v == -1e300:
+        raise RuntimeError("SpinOrbitCorrection: Unsupported period.")
+    return v
+
+
+def soc_block(mat, K, Kp, G, Gp) -> np.ndarray:
+    """2x2 spin-orbit block [[s00, s01], [s10, s11]] for reduced K = k+G, Kp = k+G'."""
+    out = np.zeros(8)
+    cm = mat.to_c()
+    args = [_d(np.ascontiguousarray(v, dtype=np.float64)) for v in (K, Kp, G, Gp)]
+    lib().orc_soc_block(C.byref(cm), *args, _d(out))
+    z = out[0::2] + 1j * out[1::2]
+    return z.reshape(2, 2)
