@@ -64,6 +64,46 @@ def cpu_baseline(mat, G, nb, ks, seconds: float):
     }
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "hbm_traffic_si_nl_n181.txt")
+
+
+def measured_traffic(mode: str, kernel: str, N: int):
+    """HBM bytes per k-point of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    if not os.path.exists(TRAFFIC_SUMMARY):
+        return None
+    for line in open(TRAFFIC_SUMMARY):
+        if line.startswith("TRAFFIC "):
+            rec = json.loads(line[len("TRAFFIC "):])
+            if rec["mode"] == mode and rec["kernel"] == kernel and rec["N"] == N:
+                return rec["bytes_per_k"]
+    return None
+
+
+def copy_calibration(mib: int, reps: int) -> int:
+    """Copies `mib` MiB device-to-device `reps` times (known traffic: read + write of that size)."""
+    from ultimateepm_amd import api
+
+    ctx = api.Context(0)
+    n = mib << 20
+    a = api.DeviceArray(ctx, (n // 8,))
+    b = api.DeviceArray(ctx, (n // 8,))
+    a.zero()
+    ctx.copy(b.ptr, a.ptr, n)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.copy(b.ptr, a.ptr, n)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"calibration": "copy_kernel", "bytes_read": n, "bytes_written": n, "launches": reps + 1,
+                      "GB_per_s": round(2 * n * reps / dt / 1e9, 1)}), flush=True)
+    a.free()
+    b.free()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,7 +115,11 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration, 10)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the k-point batch shards across GPUs by "
@@ -150,9 +194,14 @@ def main() -> int:
         "frac": round(achieved / peak, 4),
         "traffic": None,
         "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
+        "k_per_launch": round(args.nk * args.steps / max(dom_launches, 1), 1),
         "flop_per_k": tridiag_flops(r["dim"]),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stage_ms.items()},
     }
+    bpk = measured_traffic(head, dominant, N)
+    if bpk is not None:
+        roofline["traffic"] = round(bpk * roofline["k_per_launch"])
+        roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
     extra = {}
     for m, rr in res.items():
         extra[f"{m}_kpts_per_s"] = round(args.nk / (rr["elapsed"] / args.steps), 1)

@@ -162,6 +162,9 @@ int uepm_device_free(uepm_ctx* ctx, void* ptr);
 int uepm_memcpy_htod(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int uepm_memcpy_dtoh(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 int uepm_memset_device(uepm_ctx* ctx, void* dst, int value, uint64_t bytes);
+/* Device-to-device copy on `stream` (0: the default stream), asynchronous;
+ * both pointers 16-byte aligned. */
+int uepm_memcpy_dtod(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 int uepm_stream_create(uepm_ctx* ctx, void** stream);
 int uepm_stream_destroy(uepm_ctx* ctx, void* stream);
 int uepm_stream_synchronize(uepm_ctx* ctx, void* stream);

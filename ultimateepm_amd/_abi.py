@@ -96,6 +96,7 @@ def lib() -> C.CDLL:
     L.uepm_memcpy_htod.argtypes = [P, P, P, U64]
     L.uepm_memcpy_dtoh.argtypes = [P, P, P, U64]
     L.uepm_memset_device.argtypes = [P, P, C.c_int, U64]
+    L.uepm_memcpy_dtod.argtypes = [P, P, P, U64, P]
     L.uepm_stream_create.argtypes = [P, C.POINTER(P)]
     L.uepm_stream_destroy.argtypes = [P, P]
     L.uepm_stream_synchronize.argtypes = [P, P]

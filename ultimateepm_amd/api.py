@@ -60,6 +60,10 @@ class Context:
     def synchronize(self):
         check(lib().uepm_device_synchronize(self._h))
 
+    def copy(self, dst: int, src: int, nbytes: int, stream: int = 0):
+        """Device-to-device copy (both pointers 16-byte aligned), asynchronous on `stream`."""
+        check(lib().uepm_memcpy_dtod(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes, C.c_void_p(stream)))
+
     def stream(self) -> "Stream":
         return Stream(self)
 

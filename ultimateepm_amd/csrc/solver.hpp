@@ -46,4 +46,7 @@ hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                          int64_t nk, hipStream_t s, double2* dY);
 
+// Device-to-device copy (16-byte aligned pointers take the vector path).
+hipError_t launch_copy_bytes(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
+
 }  // namespace uepm

@@ -566,6 +566,16 @@ int uepm_memset_device(uepm_ctx* ctx, void* dst, int value, uint64_t bytes) {
     return UEPM_OK;
 }
 
+int uepm_memcpy_dtod(uepm_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream) {
+    if (!ctx || (bytes && (!dst || !src))) return fail(UEPM_ERR_INVALID_ARG, "uepm_memcpy_dtod: null argument");
+    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_memcpy_dtod: pointers must be 16-byte aligned");
+    if (!bytes) return UEPM_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(uepm::launch_copy_bytes(dst, src, bytes, ctx->prop.multiProcessorCount, static_cast<hipStream_t>(stream)));
+    return UEPM_OK;
+}
+
 int uepm_stream_create(uepm_ctx* ctx, void** stream) {
     if (!ctx || !stream) return fail(UEPM_ERR_INVALID_ARG, "uepm_stream_create: null argument");
     HIP_OK(hipSetDevice(ctx->device));
