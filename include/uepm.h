@@ -84,7 +84,7 @@ int uepm_ctx_destroy(uepm_ctx* ctx);
  * (src/EPP/BandStructure.cpp:26-57).  Writes up to `capacity` integer
  * triples (x,y,z) of G = i*b1 + j*b2 + k*b3, |G|^2 <= G2[n-1], in the
  * reference's i,j,k loop order.  *count receives the basis size N.
- * n outside [2, 117] -> UEPM_ERR_RUNTIME (the reference returns false and
+ * n outside [2, 116] -> UEPM_ERR_RUNTIME (the reference returns false and
  * Initialize throws runtime_error). Host-only. */
 int uepm_basis_vectors(int nearest_neighbors, int32_t* G_out, int capacity, int* count);
 

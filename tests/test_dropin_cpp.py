@@ -1,0 +1,79 @@
+"""The drop-in C++ EPP classes (ultimateepm_amd/epp/, libuepm_epp.so) driven by
+tests/cpp/test_dropin.cpp, built by the package build into
+ultimateepm_amd/_build/test_dropin.  The GPU leg compares what the classes
+return with the oracle."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from ultimateepm_amd._abi import GRADIENTS, NONLOCAL, SOC, VECTORS
+from ultimateepm_amd.material import PARAMS_DIR, load
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "ultimateepm_amd", "_build", "test_dropin")
+MATS = os.path.join(PARAMS_DIR, "materials.yaml")
+COHEN = os.path.join(PARAMS_DIR, "materials-local-cohen.yaml")
+
+
+def _bin():
+    if not os.path.exists(BIN):
+        pytest.fail(f"{BIN} is missing: run the package build first")
+    return BIN
+
+
+def test_dropin_host_side():
+    files = [MATS]
+    ref_file = "/root/reference/parameter_files/materials.yaml"  # the reference's own layout, when present
+    if os.path.exists(ref_file):
+        files.append(ref_file)
+    r = subprocess.run([_bin(), "cpu", *files], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "cpu checks: ok" in r.stdout
+
+
+@pytest.mark.gpu
+def test_dropin_band_structures_vs_oracle(oracle, tmp_path):
+    print("case: drop-in C++ classes, test_dropin gpu", flush=True)
+    r = subprocess.run([_bin(), "gpu", MATS, COHEN, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # 1. path, local Cohen-Bergstresser Si, n=10, 16 bands (raw, before AdjustValues)
+    ks, _ = oracle.kpath("LGXWKULWXKG", 200)
+    ev_ref, _, _ = oracle.solve(load("Si", COHEN), oracle.basis(10), 0, 16, ks, nthreads=8)
+    ev = np.loadtxt(tmp_path / "path_raw.txt")
+    assert np.abs(ev - ev_ref).max() <= 1e-9
+    files = [f for f in os.listdir(tmp_path) if f.startswith("EPM_Si_nb_bands_16_path_LGXWKULWXKG_size_basis_137")]
+    assert len(files) == 1
+    text = (tmp_path / files[0]).read_text().splitlines()
+    assert text[0] == "# Material Si" and text[1] == "# NBands 16" and text[2] == "# Nonlocal No"
+    assert text[5].startswith("band_0,band_1") and len(text) == 6 + 200
+    # 2. Ge non-local + SOC, n=8, 2x8 bands, gradients
+    kl = np.array([[0.1, 0.2, 0.3], [-0.4, 0.05, 0.6], [0.5, 0.5, 0.5], [0, 0, 0]])
+    ev_ref, _, gr_ref = oracle.solve(load("Ge"), oracle.basis(8), NONLOCAL | SOC | GRADIENTS | VECTORS, 16, kl)
+    ev = np.loadtxt(tmp_path / "ge_soc.txt")
+    assert np.abs(ev - ev_ref).max() <= 1e-9
+    gr = np.loadtxt(tmp_path / "ge_soc_grad.txt").reshape(4, 16, 3)
+    # gradients of degenerate levels depend on the basis chosen inside the degenerate space; compare non-degenerate
+    a = 5.65e-10
+    for q in range(4):
+        for b in range(16):
+            gaps = np.abs(ev_ref[q] - ev_ref[q, b])
+            gaps[b] = np.inf
+            if b % 2 == 0 and gaps.min() > 1e-6 and np.sort(gaps)[0] > 1e-6:
+                assert np.allclose(gr[q, b], gr_ref[q, b], atol=1e-8 * a * 10)
+    # 3. per-k Hamiltonian, GaAs non-local n=6 (N=59): all eigenpairs
+    raw = np.loadtxt(tmp_path / "gaas_all.txt", max_rows=1)
+    vec = np.loadtxt(tmp_path / "gaas_all.txt", skiprows=1, max_rows=59 * 59)
+    V = (vec[:, 0] + 1j * vec[:, 1]).reshape(59, 59).T  # column j = eigenvector j
+    k = np.array([0.13, -0.27, 0.41])
+    m = load("GaAs")
+    G = oracle.basis(6)
+    H = oracle.assemble(m, G, k, nl=True, soc=False, herm=True)
+    w = np.linalg.eigvalsh(H)
+    assert np.abs(raw - w).max() <= 1e-9
+    assert np.abs(H @ V - V * raw).max() <= 1e-8
+    assert np.abs(V.conj().T @ V - np.eye(59)).max() <= 1e-10

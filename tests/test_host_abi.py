@@ -88,8 +88,9 @@ def test_basis_counts(uepm, n, N):
 
 
 def test_basis_out_of_table(uepm):
-    """n outside [2, 117]: the reference's GenerateBasisVectors returns false (BandStructure.cpp:34-37)."""
-    for n in (0, 1, 118):
+    """n outside [2, 116] (the size of the shell table): GenerateBasisVectors returns false (BandStructure.cpp:34-37)."""
+    assert len(uepm.basis_vectors(116)) > 0
+    for n in (0, 1, 117):
         with pytest.raises(_abi.UepmError) as ei:
             uepm.basis_vectors(n)
         assert ei.value.code == _abi.UEPM_ERR_RUNTIME
@@ -108,11 +109,11 @@ def test_kpath_unknown_label(uepm):
 def test_kpath_nodes_are_exact(uepm):
     pts, pos = uepm.kpath("LGXWKULWXKG", 2000)
     assert len(pts) == 2000
-    nodes = {"L": (0.5, 0.5, 0.5), "G": (0, 0, 0), "X": (0, 1, 0), "W": (0.5, 1, 0), "K": (0.75, 0.75, 0),
-             "U": (0.25, 1, 0.25)}
+    # fcc symmetry points in units of 2 pi / a (SymmetryPoints.cpp:21-28)
+    nodes = {"L": (0.5, 0.5, 0.5), "G": (0, 0, 0), "X": (1, 0, 0), "W": (1, 0.5, 0), "K": (0.75, 0.75, 0),
+             "U": (1, 0.25, 0.25)}
     for lab, p in zip("LGXWKULWXKG", pos):
-        assert np.allclose(pts[p], nodes[lab], atol=0) or np.linalg.norm(pts[p]) == pytest.approx(
-            np.linalg.norm(nodes[lab]))
+        assert tuple(pts[p]) == nodes[lab], lab
     assert pos[0] == 0 and pos[-1] == 1999
     assert (np.diff(pos.astype(int)) > 0).all()
 

@@ -1,8 +1,9 @@
-"""Build recipe for libuepm.so (HIP kernels for gfx950 + host C ABI).
+"""Build recipe for libuepm.so (HIP kernels for gfx950 + host C ABI), the
+drop-in C++ EPP classes over it (libuepm_epp.so) and their test driver.
 
-Compiles in-tree with hipcc; the shared library lands next to this file so it
-travels with the repository snapshot.  Packed FP32 instructions are disabled
-for all device code.
+Compiles in-tree with hipcc / g++; the shared libraries land next to this file
+so they travel with the repository snapshot.  Packed FP32 instructions are
+disabled for all device code.
 """
 from __future__ import annotations
 
@@ -15,6 +16,13 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OUT = os.path.join(HERE, "libuepm.so")
 BUILD_DIR = os.path.join(HERE, "_build")
+EPP_DIR = os.path.join(HERE, "epp")
+EPP_OUT = os.path.join(HERE, "libuepm_epp.so")
+EPP_SOURCES = ["Material.cpp", "yaml_reader.cpp", "device_context.cpp", "SymmetryPoints.cpp", "Hamiltonian.cpp",
+               "BandStructure.cpp"]
+DROPIN_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_dropin.cpp")
+DROPIN_TEST_BIN = os.path.join(BUILD_DIR, "test_dropin")
+CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -45,7 +53,21 @@ def build(verbose: bool = True) -> str:
     tmp = OUT + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, OUT)
+    build_epp()
     return OUT
+
+
+def build_epp() -> str:
+    """The drop-in EPP classes (host C++ only, no device code) and their test driver."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    flags = ["-std=c++17", "-O2", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{EPP_DIR}"]
+    tmp = EPP_OUT + ".tmp"
+    _run([CXX, *flags, "-shared", "-o", tmp, *[os.path.join(EPP_DIR, f) for f in EPP_SOURCES], f"-L{HERE}", "-luepm",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, EPP_OUT)
+    _run([CXX, *flags, "-o", DROPIN_TEST_BIN, DROPIN_TEST_SRC, f"-L{HERE}", "-luepm_epp", "-luepm",
+          "-Wl,-rpath,$ORIGIN/.."])
+    return EPP_OUT
 
 
 if __name__ == "__main__":
