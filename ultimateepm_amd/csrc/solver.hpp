@@ -46,6 +46,12 @@ hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                          int64_t nk, hipStream_t s, double2* dY);
 
+// Register-resident complex64 reduction for dim <= tridiag_reg_max_n(): same
+// outputs as launch_tridiag_blocked_c64 (reflectors in Vout column-major with ld).
+int tridiag_reg_max_n();
+hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
+                                  float2* dtau, hipStream_t s);
+
 // Device-to-device copy (16-byte aligned pointers take the vector path).
 hipError_t launch_copy_bytes(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
 

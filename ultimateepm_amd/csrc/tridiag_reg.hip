@@ -1,0 +1,404 @@
+// tridiag_reg.hip — register-resident complex64 Householder reduction of a
+// Hermitian H(k) to real symmetric tridiagonal form, one 512-lane workgroup
+// per k-point, for dim <= 192 (gfx950).
+//
+// Layout.  The 512 lanes form a PR x PC = 32 x 16 grid: lane (r, c) with
+// c = tid % 16, r = tid / 16 (a wave holds four r values and all sixteen c).
+// Lane (r, c) keeps in VGPRs the lower-triangle elements A(i, j) with
+// i = r + 32 l, j = c + 16 m (cyclic in both directions, so every lane keeps
+// a share of the shrinking trailing matrix until the end).  Only blocks
+// (l, m) that can hold i >= j are stored: 42 complex values per lane at
+// dim <= 192.  The diagonal is stored halved and the strict upper part of a
+// block as zero, so the symmetric product treats every stored element alike:
+// A(i, j) feeds row i (A v_j) and column j (conj(A) v_i).
+//
+// Step s (LAPACK xHETD2, lower, restated; H_s = I - tau v v^H, v[s+1] = 1):
+//   a. the lanes of column s put x = A(s+1:, s) in LDS              | barrier
+//   b. every wave forms beta, tau, v from x (redundantly)
+//   c. y = A v: row sums across the 16 lanes of a row group (DPP), column
+//      sums across the 4 row groups of a wave (row swaps), one
+//      partial per wave in LDS; each lane also sums its share of v^H A v,
+//      so p^H v = conj(tau) v^H A v needs no separate dot product      | barrier
+//   d. y = sum of the partials, one row per lane                       | barrier
+//   e. w = tau y - (1/2)|tau|^2 (v^H A v) v; A -= v w^H + w v^H on the live blocks
+// Blocks whose columns are all <= s are skipped (one wave-uniform test per
+// block column: a stored block's rows are live while its columns are).
+// Outputs match the LDS-blocked reduction: d, e (real), tau, reflectors below
+// the sub-diagonal of column s of Vout (v[s+1] = 1 implied).
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "solver.hpp"
+
+namespace uepm {
+
+namespace {
+
+struct f2 {
+    float x, y;
+};
+__device__ __forceinline__ f2 mk(float a, float b) { return {a, b}; }
+__device__ __forceinline__ f2 add(f2 a, f2 b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ f2 mul(f2 a, f2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+// acc += a * b
+__device__ __forceinline__ void fma_ab(f2& acc, f2 a, f2 b) {
+    acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
+    acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+}
+// acc += conj(a) * b
+__device__ __forceinline__ void fma_cab(f2& acc, f2 a, f2 b) {
+    acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
+    acc.y = fmaf(a.x, b.y, fmaf(-a.y, b.x, acc.y));
+}
+
+// Reductions: DPP inside a 16-lane row, VALU row swaps across the rows of a wave.
+template <int CTRL>
+__device__ __forceinline__ float dppv(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+// sum over the 16 lanes of a row; every lane of the row gets the total
+__device__ __forceinline__ float row_sum16(float v) {
+    v += dppv<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dppv<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dppv<0x141>(v);  // row_half_mirror
+    v += dppv<0x140>(v);  // row_mirror
+    return v;
+}
+// gfx950 VALU row swaps, written out because the two-result builtins are not
+// usable here.  permlane16: a' = [a.R0 b.R0 a.R2 b.R2], b' = [a.R1 b.R1 a.R3 b.R3]
+// (R = 16-lane row); permlane32: a' = [a.lo b.lo], b' = [a.hi b.hi].  The s_nop
+// pairs keep VALU writes and reads of the operands apart from the swap.
+__device__ __forceinline__ void swap16(float& a, float& b) {
+    asm volatile("s_nop 2\n\tv_permlane16_swap_b32 %0, %1\n\ts_nop 2" : "+v"(a), "+v"(b));
+}
+__device__ __forceinline__ void swap32(float& a, float& b) {
+    asm volatile("s_nop 2\n\tv_permlane32_swap_b32 %0, %1\n\ts_nop 2" : "+v"(a), "+v"(b));
+}
+// sum over the wave, wave-uniform result
+__device__ __forceinline__ float wave_sum(float v) {
+    v = row_sum16(v);
+    float a = v, b = v;
+    swap16(a, b);
+    a += b;
+    float c = a, d = a;
+    swap32(c, d);
+    return c + d;
+}
+// Reduce-scatter of two complex column partials over the four rows of a wave:
+// returns, per lane, row 0: Re p0, row 1: Im p0, row 2: Re p1, row 3: Im p1
+// (each summed over the four rows).
+__device__ __forceinline__ float cross_rows4_pair(float p0x, float p0y, float p1x, float p1y) {
+    swap16(p0x, p0y);  // rows 0,2: Re p0 parts; rows 1,3: Im p0 parts
+    p0x += p0y;
+    swap16(p1x, p1y);
+    p1x += p1y;
+    swap32(p0x, p1x);  // lanes 0-31: p0 (Re | Im), lanes 32-63: p1 (Re | Im)
+    return p0x + p1x;
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return r;
+}
+
+// Stored blocks of the cyclic lower triangle.
+template <int PR, int PC, int LR, int LC>
+struct RegLayout {
+    static constexpr int mmax(int l) {
+        const int v = (PR * l + PR - 1) / PC;
+        return v < LC - 1 ? v : LC - 1;
+    }
+    static constexpr int base(int l) {
+        int s = 0;
+        for (int q = 0; q < l; ++q) s += mmax(q) + 1;
+        return s;
+    }
+    static constexpr int count() { return base(LR); }
+    static constexpr int idx(int l, int m) { return base(l) + m; }
+    // first block column of row block l that may hold i <= j for some lane
+    static constexpr int mcross(int l) {
+        const int t = PR * l - PC + 1;  // smallest m with PR*l - PC*m <= PC - 1
+        return t <= 0 ? 0 : (t + PC - 1) / PC;
+    }
+    static constexpr int XC = PR / PC + 1;  // crossing blocks per row block (upper bound)
+    // first row block l with block (l, m) stored
+    static constexpr int lmin(int m) {
+        int l = 0;
+        while (l < LR && mmax(l) < m) ++l;
+        return l;
+    }
+};
+
+}  // namespace
+
+template <int PR, int PC, int LR, int LC, int MINWG>
+__global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double2* __restrict__ Hin, int n, int ld,
+                                                                    float2* __restrict__ Vout,
+                                                                    double* __restrict__ dout,
+                                                                    double* __restrict__ eout,
+                                                                    float2* __restrict__ tauout) {
+    constexpr int BS = PR * PC;
+    constexpr int NW = BS / 64;
+    using Lay = RegLayout<PR, PC, LR, LC>;
+    constexpr int NS = Lay::count();
+    constexpr int XC = Lay::XC;
+    constexpr int NX = PR * LR;  // rows covered by the grid
+    constexpr int NXR = (NX + 63) / 64;
+    static_assert(PC == 16 && BS % 64 == 0, "row groups are the 16-lane DPP rows of a wave");
+    static_assert(PC * LC == NX && NX <= BS && LC % 2 == 0, "square coverage, one y entry per lane");
+
+    __shared__ f2 X[NX];       // column s: zero at rows <= s and >= n
+    __shared__ f2 XD[NX];      // column s as stored (its diagonal entry is A(s, s) / 2)
+    __shared__ f2 RP[NX];      // row sums of the symmetric product
+    __shared__ f2 CP[NW][NX];  // column sums, one row per wave
+    __shared__ f2 Y[NX];       // y = A v, zero at rows <= s and >= n
+    __shared__ float Q[NW];    // per-wave partial v^H A v
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int c = tid % PC, r = tid / PC;
+    const int64_t kk = blockIdx.x;
+    const double2* H = Hin + kk * (int64_t)ld * n;
+    float2* Vk = Vout + kk * (int64_t)ld * n;
+    double* d = dout + kk * n;
+    double* e = eout + kk * n;
+    float2* tau_o = tauout + kk * n;
+
+    // ---- load the lower triangle (diagonal halved, strict upper zero) ----
+    f2 A[NS];
+#pragma unroll
+    for (int l = 0; l < LR; ++l) {
+#pragma unroll
+        for (int m = 0; m <= Lay::mmax(l); ++m) {
+            const int i = r + PR * l, j = c + PC * m;
+            f2 a = mk(0.f, 0.f);
+            if (i < n && j < n && i >= j) {
+                const double2 h = H[i + (int64_t)j * ld];
+                const float f = (i == j) ? 0.5f : 1.0f;
+                a = mk((float)h.x * f, (float)h.y * f);
+            }
+            A[Lay::idx(l, m)] = a;
+        }
+    }
+    // update weights of the blocks that straddle the diagonal: 1 below, 1/2 on, 0 above
+    float HF[LR][XC];
+#pragma unroll
+    for (int l = 0; l < LR; ++l)
+#pragma unroll
+        for (int q = 0; q < XC; ++q) {
+            const int m = Lay::mcross(l) + q;
+            const int dd = (r - c) + PR * l - PC * m;
+            HF[l][q] = dd > 0 ? 1.f : (dd == 0 ? 0.5f : 0.f);
+        }
+    const f2 one = mk(1.f, 0.f), zero = mk(0.f, 0.f);
+
+    for (int s = 0; s + 1 < n; ++s) {
+        const int cs = s % PC, ms = s / PC;
+        const int l0 = (s + 1) / PR, m0 = (s + 1) / PC;  // first live row / column blocks
+        // ---- a. column s to LDS (every row of the grid) ----
+        const bool own = c == cs;
+#pragma unroll
+        for (int m = 0; m < LC; ++m) {
+            if (m == ms) {  // wave-uniform
+                if (own) {
+#pragma unroll
+                    for (int l = 0; l < LR; ++l) {
+                        const int i = r + PR * l;
+                        if (m <= Lay::mmax(l)) {
+                            const f2 a = A[Lay::idx(l, m)];
+                            X[i] = i > s ? a : zero;
+                            XD[i] = a;
+                        } else {
+                            X[i] = zero;  // block above the diagonal: rows < s
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- b. reflector, redundantly in every wave ----
+        float part = 0.f;
+#pragma unroll
+        for (int q = 0; q < NXR; ++q) {
+            const int i = s + 2 + lane + 64 * q;
+            const f2 a = X[i < NX ? i : NX - 1];
+            const float t = fmaf(a.x, a.x, a.y * a.y);
+            part += i < NX ? t : 0.f;
+        }
+        const double xn2 = (double)wave_sum(part);
+        const f2 al = X[s + 1];
+        const double ar = al.x, ai = al.y;
+        double beta = ar, tr = 0.0, ti = 0.0, sr = 1.0, si = 0.0;
+        if (!(xn2 == 0.0 && ai == 0.0)) {
+            const double nrm = sqrt(ar * ar + ai * ai + xn2);
+            beta = ar >= 0 ? -nrm : nrm;
+            const double ib = rcp_nr(beta);
+            tr = (beta - ar) * ib;
+            ti = -ai * ib;
+            const double br = ar - beta, iden = rcp_nr(br * br + ai * ai);
+            sr = br * iden;
+            si = -ai * iden;
+        }
+        const f2 tau = mk((float)tr, (float)ti);
+        const f2 sc = mk((float)sr, (float)si);
+        if (tid == 0) {
+            d[s] = 2.0 * (double)XD[s].x;
+            e[s] = beta;
+            tau_o[s] = make_float2(tau.x, tau.y);
+        }
+        {
+            const int i = s + 2 + tid;
+            if (i < n) {
+                const f2 v = mul(X[i], sc);
+                Vk[i + (int64_t)s * ld] = make_float2(v.x, v.y);
+            }
+        }
+        // v at this lane's rows and columns (X is zero outside s+1..n-1; loads
+        // unconditional so they issue together)
+        f2 vr[LR], vc[LC];
+#pragma unroll
+        for (int l = 0; l < LR; ++l) {
+            const int i = r + PR * l;
+            const f2 v = mul(X[i], sc);
+            vr[l] = i == s + 1 ? one : v;
+        }
+#pragma unroll
+        for (int m = 0; m < LC; ++m) {
+            const int j = c + PC * m;
+            const f2 v = mul(X[j], sc);
+            vc[m] = j == s + 1 ? one : v;
+        }
+        // ---- c. y = A v on the live blocks; local share of v^H A v ----
+        f2 rp[LR], cp[LC];
+#pragma unroll
+        for (int l = 0; l < LR; ++l) rp[l] = zero;
+#pragma unroll
+        for (int m = 0; m < LC; ++m) cp[m] = zero;
+#pragma unroll
+        for (int m = LC - 1; m >= 0; --m) {
+            if (m >= m0) {  // a stored block is live while its block column is
+#pragma unroll
+                for (int l = Lay::lmin(m); l < LR; ++l) {
+                    const f2 a = A[Lay::idx(l, m)];
+                    fma_ab(rp[l], a, vc[m]);
+                    fma_cab(cp[m], a, vr[l]);
+                }
+            }
+        }
+        // sum over stored elements of conj(v_i) A_ij v_j; v^H A v = 2 Re of its total
+        float qv = 0.f;
+#pragma unroll
+        for (int l = 0; l < LR; ++l)
+            if (l >= l0) qv = fmaf(vr[l].x, rp[l].x, fmaf(vr[l].y, rp[l].y, qv));
+#pragma unroll
+        for (int l = 0; l < LR; ++l) {
+            if (l >= l0) {
+                const float sx = row_sum16(rp[l].x), sy = row_sum16(rp[l].y);
+                if (c == 0) RP[r + PR * l] = mk(sx, sy);
+            }
+        }
+        // column sums, two block columns per reduce-scatter: lane row q holds
+        // (Re, Im) of block column m (q < 2) or m + 1 (q >= 2)
+        {
+            const int q = lane >> 4;
+            float* cpw = &CP[wv][0].x;
+#pragma unroll
+            for (int m = 0; m < LC; m += 2) {
+                if (m + 1 >= m0) {
+                    const float t = cross_rows4_pair(cp[m].x, cp[m].y, cp[m + 1].x, cp[m + 1].y);
+                    cpw[2 * (c + PC * (m + (q >> 1))) + (q & 1)] = t;
+                }
+            }
+        }
+        qv = wave_sum(qv);
+        if (lane == 0) Q[wv] = qv;
+        __syncthreads();
+        // ---- d. y = row sums + column sums of the waves, one row per lane ----
+        if (tid < NX) {
+            f2 y = zero;
+            if (tid > s && tid < n) {
+                y = RP[tid];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) y = add(y, CP[w][tid]);
+            }
+            Y[tid] = y;
+        }
+        __syncthreads();
+        // ---- e. w = tau y + alpha v, alpha = -(tau/2) p^H v = -(1/2)|tau|^2 v^H A v; rank-2 update ----
+        float vav = 0.f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) vav += Q[w];
+        const float alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
+        f2 wr[LR], wc[LC];
+#pragma unroll
+        for (int l = 0; l < LR; ++l) {
+            f2 w = mul(tau, Y[r + PR * l]);
+            w.x = fmaf(alpha, vr[l].x, w.x);
+            w.y = fmaf(alpha, vr[l].y, w.y);
+            wr[l] = w;
+        }
+#pragma unroll
+        for (int m = 0; m < LC; ++m) {
+            f2 w = mul(tau, Y[c + PC * m]);
+            w.x = fmaf(alpha, vc[m].x, w.x);
+            w.y = fmaf(alpha, vc[m].y, w.y);
+            wc[m] = w;
+        }
+#pragma unroll
+        for (int m = LC - 1; m >= 0; --m) {
+            if (m >= m0) {
+#pragma unroll
+                for (int l = Lay::lmin(m); l < LR; ++l) {
+                    // delta = -(v_i conj(w_j) + w_i conj(v_j))
+                    f2 dl;
+                    dl.x = -(vr[l].x * wc[m].x + vr[l].y * wc[m].y) - (wr[l].x * vc[m].x + wr[l].y * vc[m].y);
+                    dl.y = -(vr[l].y * wc[m].x - vr[l].x * wc[m].y) - (wr[l].y * vc[m].x - wr[l].x * vc[m].y);
+                    f2& a = A[Lay::idx(l, m)];
+                    if (m >= Lay::mcross(l)) {
+                        const float h = HF[l][m - Lay::mcross(l)];
+                        a.x = fmaf(h, dl.x, a.x);
+                        a.y = fmaf(h, dl.y, a.y);
+                    } else {
+                        a.x += dl.x;
+                        a.y += dl.y;
+                    }
+                }
+            }
+        }
+    }
+    // last diagonal element
+    {
+        const int s = n - 1;
+        const int cs = s % PC, ms = s / PC;
+#pragma unroll
+        for (int m = 0; m < LC; ++m)
+            if (m == ms)
+#pragma unroll
+                for (int l = 0; l < LR; ++l)
+                    if (m <= Lay::mmax(l) && c == cs && r + PR * l == s) d[s] = 2.0 * (double)A[Lay::idx(l, m)].x;
+        if (tid == 0) {
+            e[s] = 0.0;
+            tau_o[s] = make_float2(0.f, 0.f);
+        }
+    }
+}
+
+// n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12 (rows x columns of the 32 x 16 grid)
+int tridiag_reg_max_n() { return 192; }
+
+hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
+                                  float2* dtau, hipStream_t s) {
+    if (n < 1 || n > 192 || nk <= 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)nk), block(512);
+    if (n <= 64)
+        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 2, 4, 2>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
+    else if (n <= 128)
+        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 4, 8, 2>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
+    else
+        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 6, 12, 1>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
+    return hipGetLastError();
+}
+
+}  // namespace uepm

@@ -89,6 +89,7 @@ struct uepm_plan {
     DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
     uepm::TriPlan tp{}, tp32{};
     bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
+    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 192)
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
     double stage_ms[kNumStages] = {0, 0, 0, 0, 0};
@@ -350,6 +351,8 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     p->tp32 = uepm::tri_plan(dim, (int)sizeof(float2), ctx->lds_limit, tri_bs);
     const char* v1 = std::getenv("UEPM_TRIDIAG_V1");
     p->tri_v1 = v1 && v1[0] == '1';
+    const char* reg = std::getenv("UEPM_TRIDIAG_REG");
+    p->tri_reg = dim <= uepm::tridiag_reg_max_n() && !(reg && reg[0] == '0');
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
     if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -425,7 +428,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     if (mixed) {
         HIP_OK(p->Vf.ensure(sizeof(float2) * (size_t)n * n * chunk));
         HIP_OK(p->tauf.ensure(sizeof(float2) * (size_t)n * chunk));
-        HIP_OK(p->S32.ensure(sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk));
+        if (!p->tri_reg) HIP_OK(p->S32.ensure(sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk));
         HIP_OK(p->Ev.ensure(sizeof(double) * (size_t)nbr * chunk));
         HIP_OK(p->X0.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
         HIP_OK(p->Y.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
@@ -464,8 +467,12 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
             float2* Vc = p->Vf.as<float2>();
             HIP_OK(rec(ci, kTridiag, 0));
-            HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
-                                                    p->e.as<double>(), p->tauf.as<float2>(), s));
+            if (p->tri_reg)
+                HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, p->d.as<double>(), p->e.as<double>(),
+                                                    p->tauf.as<float2>(), s));
+            else
+                HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
+                                                        p->e.as<double>(), p->tauf.as<float2>(), s));
             HIP_OK(rec(ci, kTridiag, 1));
             HIP_OK(rec(ci, kTriEig, 0));
             HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nbr, c, p->Ev.as<double>(),
