@@ -26,7 +26,7 @@ CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip", "copy.hip", "tridiag_reg.hip"]
+DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip", "copy.hip", "tridiag_reg.hip", "backtransform.hip"]
 HOST_SOURCES = ["uepm_abi.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -90,6 +90,7 @@ struct uepm_plan {
     uepm::TriPlan tp{}, tp32{};
     bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
     bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 192)
+    bool bt_v1 = false;   // the one-wave-per-band back-transform (A/B reference)
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
     double stage_ms[kNumStages] = {0, 0, 0, 0, 0};
@@ -353,6 +354,8 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     p->tri_v1 = v1 && v1[0] == '1';
     const char* reg = std::getenv("UEPM_TRIDIAG_REG");
     p->tri_reg = dim <= uepm::tridiag_reg_max_n() && !(reg && reg[0] == '0');
+    const char* btv1 = std::getenv("UEPM_BT_V1");
+    p->bt_v1 = btv1 && btv1[0] == '1';
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
     if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -479,9 +482,16 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
                                         p->Z.as<double>(), p->W.as<double>(), 1, s));
             HIP_OK(rec(ci, kTriEig, 1));
             HIP_OK(rec(ci, kBack, 0));
-            HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, p->tauf.as<float2>(), p->d.as<double>(), p->e.as<double>(),
-                                                  p->Ev.as<double>(), p->Z.as<double>(), p->X0.as<double2>(), p->P, kc,
-                                                  p->G.as<int32_t>(), nbr, c, s));
+            if (p->bt_v1) {
+                HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, p->tauf.as<float2>(), p->d.as<double>(),
+                                                      p->e.as<double>(), p->Ev.as<double>(), p->Z.as<double>(),
+                                                      p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, s));
+            } else {
+                HIP_OK(uepm::launch_cluster_orth(p->d.as<double>(), p->e.as<double>(), p->Ev.as<double>(),
+                                                 p->Z.as<double>(), n, nbr, c, s));
+                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, p->tauf.as<float2>(), p->Z.as<double>(), nbr,
+                                                p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, s));
+            }
             HIP_OK(rec(ci, kBack, 1));
             HIP_OK(rec(ci, kRefine, 0));
             HIP_OK(uepm::launch_refine(Hc, n, n, p->X0.as<double2>(), nbr, nb, devals + c0 * nb,
@@ -506,10 +516,18 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         if (need_vec) {
             double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : p->X.as<double2>();
             HIP_OK(rec(ci, kBack, 0));
-            HIP_OK(uepm::launch_backtransform(Hc, n, n, p->tau.as<double2>(), p->d.as<double>(), p->e.as<double>(),
-                                              devals + c0 * nb, p->Z.as<double>(), Xc,
-                                              dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), nb,
-                                              c, s));
+            if (p->bt_v1) {
+                HIP_OK(uepm::launch_backtransform(Hc, n, n, p->tau.as<double2>(), p->d.as<double>(),
+                                                  p->e.as<double>(), devals + c0 * nb, p->Z.as<double>(), Xc,
+                                                  dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
+                                                  p->G.as<int32_t>(), nb, c, s));
+            } else {
+                HIP_OK(uepm::launch_cluster_orth(p->d.as<double>(), p->e.as<double>(), devals + c0 * nb,
+                                                 p->Z.as<double>(), n, nb, c, s));
+                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, p->tau.as<double2>(), p->Z.as<double>(), nb, Xc,
+                                                dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
+                                                p->G.as<int32_t>(), c, s));
+            }
             HIP_OK(rec(ci, kBack, 1));
         }
     }
