@@ -4,7 +4,8 @@
 // Input: nbr approximate lowest eigenvectors X0 of H(k) from the complex64
 // reduction (accurate to ~1e-6), the FP64 H(k) as assembled.  In FP64:
 //   Y = H X0,  S = X0^H X0,  M = X0^H Y,  S = L L^H (Cholesky),
-//   M' = L^{-1} M L^{-H} = Z Theta Z^H (cyclic Jacobi),  X1 = X0 L^{-H} Z.
+//   M' = L^{-1} M L^{-H} = Z Theta Z^H (Jacobi),  X1 = X0 L^{-H} Z.
+// For nbr <= 16, Y, S and M run on FP64 MFMA without storing Y.
 // Theta are the Ritz values of the FP64 H on span(X0): the error of each
 // is second order in the eigenvector error (~1e-11 eV for isolated bands) and
 // the Ritz step resolves mixing inside near-degenerate clusters, which a
@@ -44,6 +45,227 @@ constexpr int kChunk = 64;
 
 }  // namespace
 
+// The small Hermitian problem on nbr <= LDM Ritz vectors, in one wave:
+// Cholesky S = L L^H, M' = L^{-1} M L^{-H}, parallel-ordered Jacobi M' = Z Theta Z^H,
+// C = L^{-H} Z (in Wt), Ritz values th and the ascending order perm.
+// Inputs Ss = S, Mm = M (row-major, stride LDM); Ss, Mm, Zm, Wt are overwritten.
+template <int LDM>
+__device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* Zm, double2* Wt, double* th,
+                                    int* perm, double4* rot, int lane) {
+    const int i = lane;
+    for (int j = 0; j < nbr; ++j) {
+        double djj = Ss[j * LDM + j].x;
+        for (int k = 0; k < j; ++k) {
+            const double2 l = Ss[j * LDM + k];
+            djj -= l.x * l.x + l.y * l.y;
+        }
+        const double ljj = sqrt(djj);
+        if (i > j && i < nbr) {
+            // L[i][j] = (S[i][j] - sum_k L[i][k] conj(L[j][k])) / L[j][j]
+            double2 s = Ss[i * LDM + j];
+            for (int k = 0; k < j; ++k)
+                s = zsub(s, zmul(Ss[i * LDM + k], make_double2(Ss[j * LDM + k].x, -Ss[j * LDM + k].y)));
+            Ss[i * LDM + j] = zscale(s, 1.0 / ljj);
+        }
+        if (i == 0) Ss[j * LDM + j] = make_double2(ljj, 0.0);
+        wsync();
+    }
+    // Wt = L^{-1} M  (lane = column of M)
+    if (i < nbr) {
+        for (int r = 0; r < nbr; ++r) {
+            double2 s = Mm[r * LDM + i];
+            for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Wt[k * LDM + i]));
+            Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
+        }
+    }
+    wsync();
+    // M' = (L^{-1} Wt^H)^H  (lane = column of Wt^H, i.e. row i of Wt)
+    if (i < nbr) {
+        for (int r = 0; r < nbr; ++r) {
+            const double2 w = Wt[i * LDM + r];
+            double2 s = make_double2(w.x, -w.y);
+            for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Zm[k * LDM + i]));
+            Zm[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);  // (L^{-1} Wt^H)[r][i]
+        }
+    }
+    wsync();
+    if (i < nbr)
+        for (int c = 0; c < nbr; ++c) {
+            const double2 u = Zm[c * LDM + i];  // M'[i][c] = conj(U[c][i])
+            Mm[i * LDM + c] = make_double2(u.x, -u.y);
+        }
+    wsync();
+    if (i < nbr)
+        for (int c = 0; c < nbr; ++c) Zm[i * LDM + c] = make_double2(i == c ? 1.0 : 0.0, 0.0);
+    if (i < nbr) {
+        for (int c = i + 1; c < nbr; ++c) {  // symmetrise
+            const double2 a = Mm[i * LDM + c], b = Mm[c * LDM + i];
+            const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+            Mm[i * LDM + c] = h;
+            Mm[c * LDM + i] = make_double2(h.x, -h.y);
+        }
+        Mm[i * LDM + i].y = 0.0;
+    }
+    wsync();
+    // Jacobi in round-robin order: m/2 disjoint pairs per round, m-1 rounds per
+    // sweep (m = nbr rounded up to even; the pad index takes no rotation).  A pair
+    // is rotated while its off-diagonal element still moves an eigenvalue by more
+    // than ~1e-13 of the local scale (|c|^2 / gap) or is not negligible outright.
+    const int m = (nbr + 1) & ~1, half = m >> 1;
+    for (int sweep = 0; sweep < 20; ++sweep) {
+        int rotated = 0;
+        for (int round = 0; round < m - 1; ++round) {
+            auto elem = [&](int pos) { return pos == 0 ? 0 : ((pos - 1 + round) % (m - 1)) + 1; };
+            // rotation parameters, one lane per pair
+            int did = 0;
+            if (i < half) {
+                int p = elem(i), q = elem(m - 1 - i);
+                if (p > q) { const int t = p; p = q; q = t; }
+                double4 g = make_double4(1.0, 0.0, 1.0, 0.0);  // cs, sn, e^{-i phi}
+                if (q < nbr) {
+                    const double2 c = Mm[p * LDM + q];
+                    const double ac = sqrt(c.x * c.x + c.y * c.y);
+                    const double a = Mm[p * LDM + p].x, b = Mm[q * LDM + q].x;
+                    const double scale = fabs(a) + fabs(b) + 1e-300;
+                    const bool tiny = ac <= 1e-15 * scale || ac * ac <= 1e-13 * scale * fabs(b - a);
+                    if (!tiny) {
+                        did = 1;
+                        const double emx = c.x / ac, emy = -c.y / ac;  // e^{-i phi}, c = |c| e^{i phi}
+                        const double tau_ = (b - a) / (2.0 * ac);
+                        const double t = (tau_ >= 0 ? 1.0 : -1.0) / (fabs(tau_) + sqrt(1.0 + tau_ * tau_));
+                        const double cs = 1.0 / sqrt(1.0 + t * t);
+                        g = make_double4(cs, t * cs, emx, emy);
+                    }
+                }
+                rot[i] = g;
+            }
+            for (int o = 32; o > 0; o >>= 1) did |= __shfl_xor(did, o, 64);
+            wsync();
+            if (did) {
+                rotated = 1;
+                // columns: M <- M G, Z <- Z G with G = [[cs, sn], [-sn e, cs e]] on (p, q)
+                for (int task = i; task < half * nbr; task += 64) {
+                    const int k = task / nbr, row = task % nbr;
+                    int p = elem(k), q = elem(m - 1 - k);
+                    if (p > q) { const int t = p; p = q; q = t; }
+                    if (q >= nbr) continue;
+                    const double4 g = rot[k];
+                    const double2 em = make_double2(g.z, g.w);
+                    const double2 mp = Mm[row * LDM + p], mq = Mm[row * LDM + q];
+                    const double2 emq = zmul(em, mq);
+                    Mm[row * LDM + p] = zsub(zscale(mp, g.x), zscale(emq, g.y));
+                    Mm[row * LDM + q] = zadd(zscale(mp, g.y), zscale(emq, g.x));
+                    const double2 zp = Zm[row * LDM + p], zq = Zm[row * LDM + q];
+                    const double2 ezq = zmul(em, zq);
+                    Zm[row * LDM + p] = zsub(zscale(zp, g.x), zscale(ezq, g.y));
+                    Zm[row * LDM + q] = zadd(zscale(zp, g.y), zscale(ezq, g.x));
+                }
+                wsync();
+                // rows: M <- G^H M
+                for (int task = i; task < half * nbr; task += 64) {
+                    const int k = task / nbr, col = task % nbr;
+                    int p = elem(k), q = elem(m - 1 - k);
+                    if (p > q) { const int t = p; p = q; q = t; }
+                    if (q >= nbr) continue;
+                    const double4 g = rot[k];
+                    const double2 emc = make_double2(g.z, -g.w);  // conj(e^{-i phi})
+                    const double2 rp = Mm[p * LDM + col], rq = Mm[q * LDM + col];
+                    const double2 erq = zmul(emc, rq);
+                    Mm[p * LDM + col] = zsub(zscale(rp, g.x), zscale(erq, g.y));
+                    Mm[q * LDM + col] = zadd(zscale(rp, g.y), zscale(erq, g.x));
+                }
+                wsync();
+                if (i < half) {
+                    int p = elem(i), q = elem(m - 1 - i);
+                    if (p > q) { const int t = p; p = q; q = t; }
+                    if (q < nbr && rot[i].y != 0.0) {
+                        Mm[p * LDM + q] = make_double2(0.0, 0.0);
+                        Mm[q * LDM + p] = make_double2(0.0, 0.0);
+                        Mm[p * LDM + p].y = 0.0;
+                        Mm[q * LDM + q].y = 0.0;
+                    }
+                }
+                wsync();
+            }
+        }
+        if (!rotated) break;
+    }
+    wsync();
+    // C = L^{-H} Z (back substitution with L^H, lane = column of Z)
+    if (i < nbr) {
+        for (int r = nbr - 1; r >= 0; --r) {
+            double2 s = Zm[r * LDM + i];
+            for (int k = r + 1; k < nbr; ++k) {
+                const double2 l = Ss[k * LDM + r];  // (L^H)[r][k] = conj(L[k][r])
+                s = zsub(s, zmulc(l, Wt[k * LDM + i]));
+            }
+            Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
+        }
+    }
+    if (i < nbr) th[i] = Mm[i * LDM + i].x;
+    wsync();
+    if (i == 0) {
+        for (int b = 0; b < nbr; ++b) perm[b] = b;
+        for (int b = 0; b < nbr; ++b) {  // selection sort of the Ritz values
+            int best = b;
+            for (int c = b + 1; c < nbr; ++c)
+                if (th[perm[c]] < th[perm[best]]) best = c;
+            const int tmp = perm[b];
+            perm[b] = perm[best];
+            perm[best] = tmp;
+        }
+    }
+    wsync();
+}
+
+// X1 = X0 C[:, perm[b]] for the lowest nb Ritz pairs: eigenvalues, unit
+// eigenvectors and Hellmann-Feynman gradients (Hamiltonian.cpp:146-194).
+template <int BS, int LDM>
+__device__ void write_ritz(const double2* X, int n, int nbr, int nb, const double2* Wt, const double* th,
+                           const int* perm, int64_t kk, double* evals, double2* evecs, double* grads,
+                           const DevParams& P, const double* kpts, const int32_t* G) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid < nb) evals[kk * nb + tid] = th[perm[tid]];
+    double2* out = evecs ? evecs + kk * (int64_t)nb * n : nullptr;
+    for (int b = wv; b < nb; b += BS / 64) {
+        const int cb = perm[b];
+        double gx = 0, gy = 0, gz = 0;
+        const double* k = kpts + 3 * kk;
+        for (int r = lane; r < n; r += 64) {
+            double2 acc = make_double2(0.0, 0.0);
+            for (int a = 0; a < nbr; ++a) {
+                const double2 x = X[a * n + r];
+                const double2 c = Wt[a * LDM + cb];
+                acc.x = fma(x.x, c.x, fma(-x.y, c.y, acc.x));
+                acc.y = fma(x.x, c.y, fma(x.y, c.x, acc.y));
+            }
+            if (out) out[(int64_t)b * n + r] = acc;
+            if (grads) {
+                // Hellmann-Feynman, kinetic term (Hamiltonian.cpp:146-194); SOC: spin-down row r
+                // of basis vector r - N adds its weight to the same k + G.
+                const int i = r < P.N ? r : r - P.N;
+                const double wt = acc.x * acc.x + acc.y * acc.y;
+                gx += (k[0] + (double)G[3 * i]) * wt;
+                gy += (k[1] + (double)G[3 * i + 1]) * wt;
+                gz += (k[2] + (double)G[3 * i + 2]) * wt;
+            }
+        }
+        if (grads) {
+            for (int o = 32; o > 0; o >>= 1) {
+                gx += __shfl_xor(gx, o, 64);
+                gy += __shfl_xor(gy, o, 64);
+                gz += __shfl_xor(gz, o, 64);
+            }
+            if (lane == 0) {
+                double* g = grads + 3 * (kk * nb + b);
+                g[0] = gx * P.grad_pref * P.frac_to_phys;
+                g[1] = gy * P.grad_pref * P.frac_to_phys;
+                g[2] = gz * P.grad_pref * P.frac_to_phys;
+            }
+        }
+    }
+}
+
 template <int BS>
 __global__ __launch_bounds__(BS) void refine_kernel(const double2* __restrict__ H, int n, int ld,
                                                     double2* __restrict__ Xall, double2* __restrict__ Yall, int nbr,
@@ -57,6 +279,7 @@ __global__ __launch_bounds__(BS) void refine_kernel(const double2* __restrict__ 
     __shared__ double2 Wt[kNbrMax * kNbrMax];
     __shared__ double th[kNbrMax];
     __shared__ int perm[kNbrMax];
+    __shared__ double4 rot[kNbrMax / 2 + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t kk = blockIdx.x;
     const double2* A = H + kk * (int64_t)ld * n;
@@ -123,184 +346,120 @@ __global__ __launch_bounds__(BS) void refine_kernel(const double2* __restrict__ 
     }
     __syncthreads();
     // ---- small dense problem in wave 0
-    if (wv == 0) {
-        const int i = lane;
-        // Hermitian parts: M <- (M + M^H)/2 (M is Hermitian up to rounding)
-        // Cholesky S = L L^H, L lower, in Ss
-        for (int j = 0; j < nbr; ++j) {
-            double djj = Ss[j * kNbrMax + j].x;
-            for (int k = 0; k < j; ++k) {
-                const double2 l = Ss[j * kNbrMax + k];
-                djj -= l.x * l.x + l.y * l.y;
-            }
-            const double ljj = sqrt(djj);
-            if (i > j && i < nbr) {
-                // L[i][j] = (S[i][j] - sum_k L[i][k] conj(L[j][k])) / L[j][j]
-                double2 s = Ss[i * kNbrMax + j];
-                for (int k = 0; k < j; ++k) s = zsub(s, zmul(Ss[i * kNbrMax + k], make_double2(Ss[j * kNbrMax + k].x, -Ss[j * kNbrMax + k].y)));
-                Ss[i * kNbrMax + j] = zscale(s, 1.0 / ljj);
-            }
-            if (i == 0) Ss[j * kNbrMax + j] = make_double2(ljj, 0.0);
-            wsync();
+    if (wv == 0) small_rayleigh_ritz<kNbrMax>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+    __syncthreads();
+    write_ritz<BS, kNbrMax>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
+}
+
+// ---------------------------------------------------------------------------
+// nbr <= 16: Y = H X0 on FP64 MFMA (16x16x4), one 16-row tile of Y at a time;
+// the accumulator layout of a Y tile (lane l, item v: row 4v + l/16, band l%16)
+// is the B-operand layout of the 4-row chunks {4v..4v+3}, so S = X0^H X0 and
+// M = X0^H Y accumulate on MFMA straight from the Y accumulators and Y never
+// leaves registers.  Waves split the row tiles; their partial S, M meet in LDS.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restrict__ H, int n, int ld,
+                                                         const double2* __restrict__ Xall, int nbr, int nb,
+                                                         double* __restrict__ evals, double2* __restrict__ evecs,
+                                                         double* __restrict__ grads, DevParams P,
+                                                         const double* __restrict__ kpts,
+                                                         const int32_t* __restrict__ G) {
+    constexpr int NW = BS / 64;
+    __shared__ double part[NW][4][16][16];  // per-wave partial S_re, S_im, M_re, M_im
+    __shared__ double2 Mm[16 * 16];
+    __shared__ double2 Ss[16 * 16];
+    __shared__ double2 Zm[16 * 16];
+    __shared__ double2 Wt[16 * 16];
+    __shared__ double th[16];
+    __shared__ int perm[16];
+    __shared__ double4 rot[9];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int64_t kk = blockIdx.x;
+    const double2* A = H + kk * (int64_t)ld * n;
+    const double2* X = Xall + kk * (int64_t)nbr * n;  // X[b * n + r]
+    const bool bl = l16 < nbr;
+    const double2* Xb = X + (bl ? l16 : 0) * (int64_t)n;  // this lane's band column
+
+    dbl4 sre = {0, 0, 0, 0}, sim = sre, mre = sre, mim = sre;
+    const int ntile = (n + 15) >> 4;
+    for (int t = wv; t < ntile; t += NW) {
+        const int row = 16 * t + l16;  // A-operand row of this lane
+        dbl4 yre = {0, 0, 0, 0}, yim = yre;
+        for (int c0 = 0; c0 < n; c0 += 4) {
+            const int col = c0 + l4;
+            const bool hc = row < n && col < n;
+            const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
+            const double2 x = Xb[col < n ? col : 0];
+            const double hr = hc ? h.x : 0.0, hi = hc ? h.y : 0.0;
+            const double xr = (bl && col < n) ? x.x : 0.0, xi = (bl && col < n) ? x.y : 0.0;
+            yre = mfma64(hr, xr, yre);
+            yre = mfma64(-hi, xi, yre);
+            yim = mfma64(hr, xi, yim);
+            yim = mfma64(hi, xr, yim);
         }
-        // Wt = L^{-1} M  (lane = column of M)
-        if (i < nbr) {
-            for (int r = 0; r < nbr; ++r) {
-                double2 s = Mm[r * kNbrMax + i];
-                for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * kNbrMax + k], Wt[k * kNbrMax + i]));
-                Wt[r * kNbrMax + i] = zscale(s, 1.0 / Ss[r * kNbrMax + r].x);
-            }
+        // S, M over this tile's rows, chunk v = rows 16t + 4v .. 16t + 4v + 3
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * t + 4 * v + l4;
+            const double2 x = Xb[r < n ? r : 0];
+            const double xr = (bl && r < n) ? x.x : 0.0, xi = (bl && r < n) ? x.y : 0.0;
+            sre = mfma64(xr, xr, sre);
+            sre = mfma64(xi, xi, sre);
+            sim = mfma64(xr, xi, sim);
+            sim = mfma64(-xi, xr, sim);
+            mre = mfma64(xr, yre[v], mre);
+            mre = mfma64(xi, yim[v], mre);
+            mim = mfma64(xr, yim[v], mim);
+            mim = mfma64(-xi, yre[v], mim);
         }
-        wsync();
-        // M' = (L^{-1} Wt^H)^H  (lane = column of Wt^H, i.e. row i of Wt); symmetrised
-        if (i < nbr) {
-            for (int r = 0; r < nbr; ++r) {
-                const double2 w = Wt[i * kNbrMax + r];
-                double2 s = make_double2(w.x, -w.y);
-                for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * kNbrMax + k], Zm[k * kNbrMax + i]));
-                Zm[r * kNbrMax + i] = zscale(s, 1.0 / Ss[r * kNbrMax + r].x);  // (L^{-1} Wt^H)[r][i]
-            }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = 4 * v + l4;  // D row (band a), column l16 (band b)
+        part[wv][0][r][l16] = sre[v];
+        part[wv][1][r][l16] = sim[v];
+        part[wv][2][r][l16] = mre[v];
+        part[wv][3][r][l16] = mim[v];
+    }
+    __syncthreads();
+    for (int e = tid; e < 256; e += BS) {
+        const int a = e >> 4, b = e & 15;
+        double s0 = 0, s1 = 0, m0 = 0, m1 = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            s0 += part[w][0][a][b];
+            s1 += part[w][1][a][b];
+            m0 += part[w][2][a][b];
+            m1 += part[w][3][a][b];
         }
-        wsync();
-        if (i < nbr)
-            for (int c = 0; c < nbr; ++c) {
-                const double2 u = Zm[c * kNbrMax + i];  // M'[i][c] = conj(U[c][i])
-                Mm[i * kNbrMax + c] = make_double2(u.x, -u.y);
-            }
-        wsync();
-        if (i < nbr)
-            for (int c = 0; c < nbr; ++c) Zm[i * kNbrMax + c] = make_double2(i == c ? 1.0 : 0.0, 0.0);
-        if (i < nbr) {
-            for (int c = i + 1; c < nbr; ++c) {  // symmetrise
-                const double2 a = Mm[i * kNbrMax + c], b = Mm[c * kNbrMax + i];
-                const double2 h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-                Mm[i * kNbrMax + c] = h;
-                Mm[c * kNbrMax + i] = make_double2(h.x, -h.y);
-            }
-            Mm[i * kNbrMax + i].y = 0.0;
-        }
-        wsync();
-        // cyclic complex Jacobi: rotate while the off-diagonal element moves an
-        // eigenvalue by more than 1e-14 relative (second order |c|^2 / gap).
-        for (int sweep = 0; sweep < 12; ++sweep) {
-            int rotated = 0;
-            for (int p = 0; p < nbr - 1; ++p)
-                for (int q = p + 1; q < nbr; ++q) {
-                    const double2 c = Mm[p * kNbrMax + q];
-                    const double ac = sqrt(c.x * c.x + c.y * c.y);
-                    const double a = Mm[p * kNbrMax + p].x, b = Mm[q * kNbrMax + q].x;
-                    const double scale = fabs(a) + fabs(b) + 1e-300;
-                    if (ac <= 1e-15 * scale) continue;
-                    rotated = 1;
-                    // e^{-i phi} with c = |c| e^{i phi}
-                    const double2 em = make_double2(c.x / ac, -c.y / ac);
-                    const double tau_ = (b - a) / (2.0 * ac);
-                    const double t = (tau_ >= 0 ? 1.0 : -1.0) / (fabs(tau_) + sqrt(1.0 + tau_ * tau_));
-                    const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
-                    // G = [[cs, sn], [-sn e^{-i phi}, cs e^{-i phi}]]; M <- G^H M G, Z <- Z G
-                    if (i < nbr) {
-                        const double2 mp = Mm[i * kNbrMax + p], mq = Mm[i * kNbrMax + q];
-                        const double2 emq = zmul(em, mq);
-                        Mm[i * kNbrMax + p] = zsub(zscale(mp, cs), zscale(emq, sn));
-                        Mm[i * kNbrMax + q] = zadd(zscale(mp, sn), zscale(emq, cs));
-                        const double2 zp = Zm[i * kNbrMax + p], zq = Zm[i * kNbrMax + q];
-                        const double2 ezq = zmul(em, zq);
-                        Zm[i * kNbrMax + p] = zsub(zscale(zp, cs), zscale(ezq, sn));
-                        Zm[i * kNbrMax + q] = zadd(zscale(zp, sn), zscale(ezq, cs));
-                    }
-                    wsync();
-                    if (i < nbr) {
-                        const double2 rp = Mm[p * kNbrMax + i], rq = Mm[q * kNbrMax + i];
-                        const double2 erq = zmul(make_double2(em.x, -em.y), rq);  // conj(e^{-i phi}) rq
-                        Mm[p * kNbrMax + i] = zsub(zscale(rp, cs), zscale(erq, sn));
-                        Mm[q * kNbrMax + i] = zadd(zscale(rp, sn), zscale(erq, cs));
-                    }
-                    wsync();
-                    if (i == 0) {
-                        Mm[p * kNbrMax + q] = make_double2(0.0, 0.0);
-                        Mm[q * kNbrMax + p] = make_double2(0.0, 0.0);
-                        Mm[p * kNbrMax + p].y = 0.0;
-                        Mm[q * kNbrMax + q].y = 0.0;
-                    }
-                    wsync();
-                }
-            if (!rotated) break;
-        }
-        wsync();
-        // C = L^{-H} Z (back substitution with L^H, lane = column of Z)
-        if (i < nbr) {
-            for (int r = nbr - 1; r >= 0; --r) {
-                double2 s = Zm[r * kNbrMax + i];
-                for (int k = r + 1; k < nbr; ++k) {
-                    const double2 l = Ss[k * kNbrMax + r];  // (L^H)[r][k] = conj(L[k][r])
-                    s = zsub(s, zmulc(l, Wt[k * kNbrMax + i]));
-                }
-                Wt[r * kNbrMax + i] = zscale(s, 1.0 / Ss[r * kNbrMax + r].x);
-            }
-        }
-        if (i < nbr) th[i] = Mm[i * kNbrMax + i].x;
-        wsync();
-        if (i == 0) {
-            for (int b = 0; b < nbr; ++b) perm[b] = b;
-            for (int b = 0; b < nbr; ++b) {  // selection sort of the Ritz values
-                int best = b;
-                for (int c = b + 1; c < nbr; ++c)
-                    if (th[perm[c]] < th[perm[best]]) best = c;
-                const int tmp = perm[b];
-                perm[b] = perm[best];
-                perm[best] = tmp;
-            }
+        if (a < nbr && b < nbr) {
+            Ss[a * 16 + b] = make_double2(s0, s1);
+            Mm[a * 16 + b] = make_double2(m0, m1);
         }
     }
     __syncthreads();
-    // ---- X1 = X0 C[:, perm[b]] for the lowest nb; outputs
-    if (tid < nb) evals[kk * nb + tid] = th[perm[tid]];
-    double2* out = evecs ? evecs + kk * (int64_t)nb * n : nullptr;
-    for (int b = wv; b < nb; b += BS / 64) {
-        const int cb = perm[b];
-        double gx = 0, gy = 0, gz = 0;
-        const double* k = kpts + 3 * kk;
-        for (int r = lane; r < n; r += 64) {
-            double2 acc = make_double2(0.0, 0.0);
-            for (int a = 0; a < nbr; ++a) {
-                const double2 x = X[a * n + r];
-                const double2 c = Wt[a * kNbrMax + cb];
-                acc.x = fma(x.x, c.x, fma(-x.y, c.y, acc.x));
-                acc.y = fma(x.x, c.y, fma(x.y, c.x, acc.y));
-            }
-            if (out) out[(int64_t)b * n + r] = acc;
-            if (grads) {
-                // Hellmann-Feynman, kinetic term (Hamiltonian.cpp:146-194); SOC: spin-down row r
-                // of basis vector r - N adds its weight to the same k + G.
-                const int i = r < P.N ? r : r - P.N;
-                const double wt = acc.x * acc.x + acc.y * acc.y;
-                gx += (k[0] + (double)G[3 * i]) * wt;
-                gy += (k[1] + (double)G[3 * i + 1]) * wt;
-                gz += (k[2] + (double)G[3 * i + 2]) * wt;
-            }
-        }
-        if (grads) {
-            for (int o = 32; o > 0; o >>= 1) {
-                gx += __shfl_xor(gx, o, 64);
-                gy += __shfl_xor(gy, o, 64);
-                gz += __shfl_xor(gz, o, 64);
-            }
-            if (lane == 0) {
-                double* g = grads + 3 * (kk * nb + b);
-                g[0] = gx * P.grad_pref * P.frac_to_phys;
-                g[1] = gy * P.grad_pref * P.frac_to_phys;
-                g[2] = gz * P.grad_pref * P.frac_to_phys;
-            }
-        }
-    }
+    if (wv == 0) small_rayleigh_ritz<16>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+    __syncthreads();
+    write_ritz<BS, 16>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
 }
 
 hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr, int nb, double* devals,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                          int64_t nk, hipStream_t s, double2* dY) {
     if (nbr > kNbrMax || nb > nbr) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(refine_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, dY, nbr, nb,
-                       devals, devecs, dgrads, P, dk, dG);
+    if (nbr <= 16)
+        hipLaunchKernelGGL(refine_mfma_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
+                           nb, devals, devecs, dgrads, P, dk, dG);
+    else
+        hipLaunchKernelGGL(refine_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, dY, nbr,
+                           nb, devals, devecs, dgrads, P, dk, dG);
     return hipGetLastError();
 }
 
