@@ -377,6 +377,179 @@ __global__ void tri_eig_kernel(const double* __restrict__ dall, const double* __
 #undef IDX
 }
 
+// Wave-per-k variant: the d, e of a k-point are wave-uniform (scalar loads,
+// SGPR operands); lanes = bands x S shifts (multisection: one sweep narrows the
+// bracket of every band S+1-fold); inverse iteration on the first lane of a band.
+__device__ __forceinline__ double rcp_nr64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
+// number of eigenvalues of T below x (Sturm count, pivmin-guarded)
+__device__ __forceinline__ int sturm_count_u(const double* __restrict__ d, const double* __restrict__ e, int n,
+                                             double x, double pivmin) {
+    double q = d[0] - x;
+    if (fabs(q) < pivmin) q = -pivmin;
+    int c = q < 0.0;
+    int i = 1;
+    for (; i + 8 <= n; i += 8) {
+        double dd[8], ee[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            dd[u] = d[i + u];
+            ee[u] = e[i + u - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            q = (dd[u] - x) - (ee[u] * ee[u]) * rcp_nr64(q);
+            if (fabs(q) < pivmin) q = -pivmin;
+            c += q < 0.0;
+        }
+    }
+    for (; i < n; ++i) {
+        q = (d[i] - x) - (e[i - 1] * e[i - 1]) * rcp_nr64(q);
+        if (fabs(q) < pivmin) q = -pivmin;
+        c += q < 0.0;
+    }
+    return c;
+}
+
+template <int S>
+__global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restrict__ dall,
+                                                          const double* __restrict__ eall, int n, int nb, int64_t nk,
+                                                          double* __restrict__ evals, double* __restrict__ Z,
+                                                          double* __restrict__ W, int want_vectors) {
+    constexpr int NBW = 64 / S;  // bands per wave
+    const int64_t kk = blockIdx.x;
+    const int lane = threadIdx.x, sub = lane % S;
+    const int b = blockIdx.y * NBW + lane / S;
+    const bool live = b < nb;
+    const double* d = dall + kk * n;
+    const double* e = eall + kk * n;
+    // Gershgorin interval and scale (wave-uniform)
+    double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0, tnorm = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
+        const double er = i + 1 < n ? fabs(e[i]) : 0.0;
+        lo = fmin(lo, d[i] - el - er);
+        hi = fmax(hi, d[i] + el + er);
+        emax2 = fmax(emax2, er * er);
+        tnorm = fmax(tnorm, fabs(d[i]) + el + er);
+    }
+    const double pivmin = DBL_MIN * fmax(1.0, emax2);
+    const double eps = DBL_EPSILON;
+    lo -= 2.0 * eps * tnorm + pivmin;
+    hi += 2.0 * eps * tnorm + pivmin;
+    // multisection: lane `sub` of a band probes lo + (sub+1) (hi-lo)/(S+1)
+    for (int it = 0; it < 200; ++it) {
+        const double w = hi - lo;
+        const bool done = !live || w <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+        if (__all(done)) break;
+        const double x = lo + (double)(sub + 1) * (w / (double)(S + 1));
+        const int cnt = done ? 0 : sturm_count_u(d, e, n, x, pivmin);
+        // gather the S probes of this band
+        double nlo = lo, nhi = hi;
+#pragma unroll
+        for (int u = 0; u < S; ++u) {
+            const int cu = __shfl(cnt, lane - sub + u, 64);
+            const double xu = __shfl(x, lane - sub + u, 64);
+            if (cu > b) nhi = fmin(nhi, xu);
+            else nlo = fmax(nlo, xu);
+        }
+        if (!done) {
+            if (nlo >= nhi || nlo <= lo && nhi >= hi) {  // no progress possible (rounding): stop here
+                lo = hi = 0.5 * (lo + hi);
+            } else {
+                lo = nlo;
+                hi = nhi;
+            }
+        }
+    }
+    const double lam = 0.5 * (lo + hi);
+    if (live && sub == 0) evals[kk * nb + b] = lam;
+    if (!want_vectors || !live || sub != 0) return;
+
+    // Inverse iteration with an LU factorisation of T - lam I with partial
+    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|); the work arrays
+    // are interleaved over the nk * nb (k, band) pairs.
+    const int64_t T = nk * nb;
+    const int64_t t = kk * nb + b;
+    double* L = W;
+    double* U0 = W + (size_t)n * T;      // reciprocal pivots
+    double* U1 = W + (size_t)2 * n * T;  // first super-diagonal
+    double* U2 = W + (size_t)3 * n * T;  // second super-diagonal
+    double* PV = W + (size_t)4 * n * T;  // interchange flags
+    double* X = W + (size_t)5 * n * T;   // iterate
+#define IDX(i) ((size_t)(i) * T + t)
+    const double tiny = eps * tnorm;
+    {
+        double a = d[0] - lam;
+        double c = n > 1 ? e[0] : 0.0;
+        for (int k = 0; k + 1 < n; ++k) {
+            const double sub_ = e[k];
+            const double dn = d[k + 1] - lam;
+            const double en = (k + 2 < n) ? e[k + 1] : 0.0;
+            if (fabs(a) >= fabs(sub_)) {
+                if (a == 0.0) a = tiny;
+                const double ra = rcp_nr64(a);
+                const double mult = sub_ * ra;
+                L[IDX(k)] = mult;
+                U0[IDX(k)] = ra;
+                U1[IDX(k)] = c;
+                U2[IDX(k)] = 0.0;
+                PV[IDX(k)] = 0.0;
+                a = dn - mult * c;
+                c = en;
+            } else {
+                const double rs = rcp_nr64(sub_);
+                const double mult = a * rs;
+                L[IDX(k)] = mult;
+                U0[IDX(k)] = rs;
+                U1[IDX(k)] = dn;
+                U2[IDX(k)] = en;
+                PV[IDX(k)] = 1.0;
+                a = c - mult * dn;
+                c = -mult * en;
+            }
+        }
+        if (a == 0.0) a = tiny;
+        U0[IDX(n - 1)] = rcp_nr64(a);
+    }
+    for (int i = 0; i < n; ++i) X[IDX(i)] = lane_rand((uint32_t)i, (uint32_t)b);
+    for (int iter = 0; iter < 3; ++iter) {
+        for (int k = 0; k + 1 < n; ++k) {
+            double yk = X[IDX(k)], yk1 = X[IDX(k + 1)];
+            if (PV[IDX(k)] != 0.0) {
+                const double tmp = yk;
+                yk = yk1;
+                yk1 = tmp;
+            }
+            yk1 -= L[IDX(k)] * yk;
+            X[IDX(k)] = yk;
+            X[IDX(k + 1)] = yk1;
+        }
+        double x2 = 0.0, x1 = X[IDX(n - 1)] * U0[IDX(n - 1)];
+        X[IDX(n - 1)] = x1;
+        double amax = fabs(x1);
+        for (int k = n - 2; k >= 0; --k) {
+            const double xk = (X[IDX(k)] - U1[IDX(k)] * x1 - U2[IDX(k)] * x2) * U0[IDX(k)];
+            X[IDX(k)] = xk;
+            x2 = x1;
+            x1 = xk;
+            amax = fmax(amax, fabs(xk));
+        }
+        const double sc = 1.0 / amax;
+        for (int i = 0; i < n; ++i) X[IDX(i)] *= sc;
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) nrm += X[IDX(i)] * X[IDX(i)];
+    const double sn = 1.0 / sqrt(nrm);
+    double* z = Z + (size_t)t * n;
+    for (int i = 0; i < n; ++i) z[i] = X[IDX(i)] * sn;
+#undef IDX
+}
+
 // ------------------------------------------- (d) back-transform + gradients
 // One workgroup per k, 4 waves; wave w owns bands b = w, w+4, ...
 // Z: nk x nb x n real eigenvectors of T (unit norm).  X (output, complex):
@@ -509,10 +682,15 @@ hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, do
 
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
                           double* dW, int want_vectors, hipStream_t s) {
-    const int64_t T = nk * nb;
-    const int bs = 64;
-    hipLaunchKernelGGL(tri_eig_kernel, dim3((unsigned)((T + bs - 1) / bs)), dim3(bs), 0, s, dd, de, n, nb, nk, devals,
-                       dZ, dW, want_vectors);
+    if (nb <= 16)
+        hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
+                           dZ, dW, want_vectors);
+    else if (nb <= 32)
+        hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
+                           dZ, dW, want_vectors);
+    else
+        hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), 0, s, dd,
+                           de, n, nb, nk, devals, dZ, dW, want_vectors);
     return hipGetLastError();
 }
 
