@@ -153,6 +153,79 @@ int uepm_plan_set_timing(uepm_plan* plan, int enable);
 int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t* launches,
                           const char** name);
 
+/* ---- configuration, tuning and workspace -----------------------------------
+ * No counterpart in the reference (its solve has no tunables). */
+typedef struct uepm_config {
+    int32_t tridiag;   /* 0 automatic; 1 LDS-blocked reduction; 2 register-resident
+                          complex64 reduction (UEPM_MIXED plans, dim <= 192)          */
+    int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = auto) */
+    int32_t bt_bands;  /* bands per back-transform workgroup: 4, 8 or 16 (0 = 16)    */
+    int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */
+} uepm_config;
+int uepm_plan_get_config(const uepm_plan* plan, uepm_config* cfg);
+/* Values outside the ranges above -> UEPM_ERR_INVALID_ARG (plan unchanged). */
+int uepm_plan_set_config(uepm_plan* plan, const uepm_config* cfg);
+/* Autotuner: times every candidate configuration on the device k-points
+ * d_k[0..nk) (one warm-up and two timed solves each, on `stream`), keeps the
+ * fastest in the plan and returns it in *chosen (may be NULL). */
+int uepm_plan_tune(uepm_plan* plan, const double* d_k, int64_t nk, void* stream, uepm_config* chosen);
+/* Device workspace that solves of up to nk k-points need, and its allocation
+ * ahead of time: after uepm_plan_reserve(plan, nk) such solves allocate nothing
+ * (required before they are captured into a graph). */
+int uepm_plan_workspace_bytes(const uepm_plan* plan, int64_t nk, uint64_t* bytes);
+int uepm_plan_reserve(uepm_plan* plan, int64_t nk);
+
+/* ---- batch graphs ----------------------------------------------------------
+ * Solves of many plans (materials, cutoffs, SOC on/off: uneven dims) captured
+ * once into one hipGraph and replayed with one launch.  Items of one plan run
+ * in order on one branch; plans are spread over `branches` concurrent branches,
+ * heaviest first.  Every plan's workspace is reserved at creation, so the graph
+ * allocates nothing.  Plans must not be used elsewhere while the graph runs.
+ * No counterpart in the reference (its k loop is per material). */
+typedef struct uepm_batch_item {
+    uepm_plan* plan;
+    const double* d_k;  /* nk x 3, device */
+    int64_t nk;
+    double* d_evals;    /* nk x nb, device */
+    double* d_evecs;    /* NULL or nk x nb x dim x 2 (plan with UEPM_VECTORS) */
+    double* d_grads;    /* NULL or nk x nb x 3 (plan with UEPM_GRADIENTS) */
+} uepm_batch_item;
+typedef struct uepm_batch uepm_batch;
+int uepm_batch_create(uepm_ctx* ctx, const uepm_batch_item* items, int nitems, int branches, uepm_batch** out);
+int uepm_batch_launch(uepm_batch* batch, void* stream);
+int uepm_batch_destroy(uepm_batch* batch);
+
+/* ---- serving ----------------------------------------------------------------
+ * Continuous batching of band-structure requests on one device: a scheduler
+ * thread admits queued requests in arrival order while an in-flight slot is
+ * free and the plan workspaces fit `hbm_budget_bytes`; every admitted request
+ * runs on its own stream, so requests overlap on the device.  Plans are pooled
+ * per (material, basis, flags, bands) with resident workspaces; idle plans are
+ * evicted oldest first when the budget needs room.  A request that cannot fit
+ * even alone fails with UEPM_ERR_NO_MEMORY.  Eigenvalues only (no vectors).
+ * No counterpart in the reference. */
+typedef struct uepm_request {
+    const uepm_material* material;
+    int32_t nearest_neighbors; /* basis shells, as in uepm_basis_vectors     */
+    uint32_t flags;            /* UEPM_NONLOCAL | UEPM_SOC | UEPM_MIXED       */
+    int32_t nb;
+    const double* k;           /* host, nk x 3 (copied at submission)        */
+    int64_t nk;
+} uepm_request;
+typedef struct uepm_server_stats {
+    uint64_t submitted, completed, kpoints, evictions, used_bytes;
+    int32_t inflight, queued;
+} uepm_server_stats;
+typedef struct uepm_server uepm_server;
+int uepm_server_create(uepm_ctx* ctx, uint64_t hbm_budget_bytes, int max_inflight, uepm_server** out);
+int uepm_server_submit(uepm_server* server, const uepm_request* request, uint64_t* ticket);
+/* Blocks until the request is done: evals (nk x nb, may be NULL) and the
+ * latency from submission to completion, and the part spent queued. */
+int uepm_server_wait(uepm_server* server, uint64_t ticket, double* evals, double* latency_s, double* queue_s);
+int uepm_server_stats_get(uepm_server* server, uepm_server_stats* stats);
+/* Drains the queue and the running requests, then frees everything. */
+int uepm_server_destroy(uepm_server* server);
+
 /* ---- device plumbing ------------------------------------------------------
  * Device memory, streams and synchronisation on the context's device, for
  * callers without a GPU runtime of their own (scripts, tests, FFI hosts).

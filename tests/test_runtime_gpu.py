@@ -1,0 +1,130 @@
+"""Runtime on the GPU: plan configuration and autotuner, batch graphs over
+uneven plans, and the continuous-batching server (uepm_server_*)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from ultimateepm_amd import api, serving
+from ultimateepm_amd._abi import UepmError
+from ultimateepm_amd.material import load
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-4  # eV: the precision form's bound against FP64
+
+
+def _dev(ctx, a):
+    return api.DeviceArray.from_host(ctx, np.ascontiguousarray(a))
+
+
+def test_config_validation_and_workspace(gpu_ctx):
+    print("case: config round trip Si n=12 mixed", flush=True)
+    p = api.Plan(gpu_ctx, load("Si"), api.basis_vectors(12), api.NONLOCAL | api.MIXED, 12)
+    assert p.config() == {"tridiag": 0, "tri_block": 0, "bt_bands": 0, "max_chunk": 0}
+    p.set_config(tridiag=1, tri_block=256, bt_bands=8, max_chunk=512)
+    assert p.config() == {"tridiag": 1, "tri_block": 256, "bt_bands": 8, "max_chunk": 512}
+    for bad in ({"tridiag": 3}, {"tri_block": 128}, {"bt_bands": 5}, {"max_chunk": -1}):
+        with pytest.raises(UepmError):
+            p.set_config(**bad)
+    assert p.config()["max_chunk"] == 512  # unchanged by a rejected config
+    assert p.workspace_bytes(100) < p.workspace_bytes(512) == p.workspace_bytes(5000)
+    q = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 274
+    with pytest.raises(UepmError):
+        q.set_config(tridiag=2)  # register-resident reduction is for dim <= 192
+    p.close()
+    q.close()
+
+
+def test_tuned_plan_keeps_results(gpu_ctx):
+    print("case: autotune Si n=12 mixed, 1024 k", flush=True)
+    ks = api.bz_sample(7, 1024)
+    p = api.Plan(gpu_ctx, load("Si"), api.basis_vectors(12), api.NONLOCAL | api.MIXED, 12)
+    ev0, _, _ = p.solve(ks)
+    dk = _dev(gpu_ctx, ks)
+    chosen = p.tune(dk.ptr, len(ks))
+    assert chosen["tridiag"] in (1, 2) and chosen["bt_bands"] in (8, 16)
+    ev1, _, _ = p.solve(ks)
+    assert np.abs(ev1 - ev0).max() <= 1e-8
+    p.close()
+
+
+def test_batch_graph_matches_individual_solves(gpu_ctx):
+    print("case: batch graph over 4 uneven plans, 2 replays", flush=True)
+    specs = [("Si", 8, api.NONLOCAL | api.MIXED, 12), ("Ge", 8, api.NONLOCAL | api.SOC | api.MIXED, 16),
+             ("GaAs", 6, api.NONLOCAL | api.SOC, 16), ("InP", 10, api.NONLOCAL, 8), ("Si", 12, api.MIXED, 12)]
+    ctx = gpu_ctx
+    plans, ks, dks, devs, ref = [], [], [], [], []
+    for i, (sym, n, flags, nb) in enumerate(specs):
+        p = api.Plan(ctx, load(sym), api.basis_vectors(n), flags, nb)
+        k = api.bz_sample(100 + i, 300 + 50 * i)
+        ref.append(p.solve(k)[0])
+        plans.append(p)
+        ks.append(k)
+        dks.append(_dev(ctx, k))
+        devs.append(api.DeviceArray(ctx, (len(k), nb)))
+    b = api.Batch(ctx, [(p, dk.ptr, len(k), de.ptr) for p, dk, k, de in zip(plans, dks, ks, devs)], branches=3)
+    for replay in range(2):
+        for de in devs:
+            de.zero()
+        b.launch()
+        ctx.synchronize()
+        for p, k in zip(plans, ks):  # eager work between the replays
+            p.solve(k[:5])
+        for i, de in enumerate(devs):
+            assert np.abs(de.numpy() - ref[i]).max() <= 1e-12, (replay, specs[i])
+    b.close()
+    for p in plans:
+        p.close()
+
+
+def _workload(seed, count):
+    return serving.poisson_workload(seed, count, rate=400.0, n_min=6, n_max=10, k_max=256)
+
+
+def test_server_result_alone_equals_result_in_the_mix(gpu_ctx):
+    print("case: server alone vs mix, 24 requests", flush=True)
+    wl = _workload(11, 24)
+    srv = api.Server(gpu_ctx, 8 << 30, max_inflight=6)
+    mix, summary = serving.run(srv, wl, mixed=True, realtime=False)
+    assert summary["requests"] == 24
+    for i in (0, 5, 17, 23):
+        alone, _ = serving.run(srv, [wl[i]], mixed=True, realtime=False)
+        assert np.abs(alone[0][0] - mix[i][0]).max() <= 1e-12, i
+    st = srv.stats()
+    assert st["completed"] == 28 and st["queued"] == 0 and st["inflight"] == 0
+    srv.close()
+
+
+def test_server_mixed_vs_fp64_under_load(gpu_ctx):
+    print("case: server mixed vs FP64, 40 requests at 400 req/s", flush=True)
+    wl = _workload(12, 40)
+    srv = api.Server(gpu_ctx, 8 << 30, max_inflight=8)
+    r64, s64 = serving.run(srv, wl, mixed=False, realtime=True)
+    rmx, smx = serving.run(srv, wl, mixed=True, realtime=True)
+    worst = max(np.abs(a[0] - b[0]).max() for a, b in zip(r64, rmx))
+    print(f"mixed vs FP64 max |dE| = {worst * 1e3:.3e} meV; p99 {s64['latency_p99_ms']} / {smx['latency_p99_ms']} ms")
+    assert worst <= MIXED_TOL
+    assert smx["latency_p50_ms"] > 0 and smx["latency_p99_ms"] >= smx["latency_p50_ms"]
+    srv.close()
+
+
+def test_server_errors_and_budget(gpu_ctx):
+    print("case: server errors (InP SOC, budget)", flush=True)
+    srv = api.Server(gpu_ctx, 8 << 30, max_inflight=2)
+    k = api.bz_sample(3, 10)
+    t_bad = srv.submit(load("InP"), 6, api.NONLOCAL | api.SOC, 8, k)
+    t_ok = srv.submit(load("Si"), 6, api.NONLOCAL, 8, k)
+    with pytest.raises(UepmError) as ei:
+        srv.wait(t_bad)
+    assert "Unsupported period" in str(ei.value)
+    ev, lat, _ = srv.wait(t_ok)
+    assert ev.shape == (10, 8) and lat > 0
+    srv.close()
+    # 64 KiB: less than one k-point's workspace at N=181 (a 1 MiB budget still fits one point per chunk)
+    tiny = api.Server(gpu_ctx, 64 << 10, max_inflight=1)
+    t = tiny.submit(load("Si"), 12, api.NONLOCAL, 8, k)
+    with pytest.raises(UepmError) as ei:
+        tiny.wait(t)
+    assert ei.value.code == 6  # UEPM_ERR_NO_MEMORY
+    tiny.close()

@@ -56,6 +56,28 @@ class uepm_material(C.Structure):
     ]
 
 
+class uepm_config(C.Structure):
+    """Mirror of ``uepm_config``: 0 fields mean automatic."""
+
+    _fields_ = [("tridiag", C.c_int32), ("tri_block", C.c_int32), ("bt_bands", C.c_int32), ("max_chunk", C.c_int32)]
+
+
+class uepm_batch_item(C.Structure):
+    _fields_ = [("plan", C.c_void_p), ("d_k", C.c_void_p), ("nk", C.c_int64), ("d_evals", C.c_void_p),
+                ("d_evecs", C.c_void_p), ("d_grads", C.c_void_p)]
+
+
+class uepm_request(C.Structure):
+    _fields_ = [("material", C.POINTER(uepm_material)), ("nearest_neighbors", C.c_int32), ("flags", C.c_uint32),
+                ("nb", C.c_int32), ("k", C.POINTER(C.c_double)), ("nk", C.c_int64)]
+
+
+class uepm_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("kpoints", C.c_uint64),
+                ("evictions", C.c_uint64), ("used_bytes", C.c_uint64), ("inflight", C.c_int32),
+                ("queued", C.c_int32)]
+
+
 class UepmError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"uepm error {code}: {msg}")
@@ -101,6 +123,19 @@ def lib() -> C.CDLL:
     L.uepm_stream_destroy.argtypes = [P, P]
     L.uepm_stream_synchronize.argtypes = [P, P]
     L.uepm_device_synchronize.argtypes = [P]
+    L.uepm_plan_get_config.argtypes = [P, C.POINTER(uepm_config)]
+    L.uepm_plan_set_config.argtypes = [P, C.POINTER(uepm_config)]
+    L.uepm_plan_tune.argtypes = [P, P, I64, P, C.POINTER(uepm_config)]
+    L.uepm_plan_workspace_bytes.argtypes = [P, I64, C.POINTER(U64)]
+    L.uepm_plan_reserve.argtypes = [P, I64]
+    L.uepm_batch_create.argtypes = [P, C.POINTER(uepm_batch_item), C.c_int, C.c_int, C.POINTER(P)]
+    L.uepm_batch_launch.argtypes = [P, P]
+    L.uepm_batch_destroy.argtypes = [P]
+    L.uepm_server_create.argtypes = [P, U64, C.c_int, C.POINTER(P)]
+    L.uepm_server_submit.argtypes = [P, C.POINTER(uepm_request), C.POINTER(U64)]
+    L.uepm_server_wait.argtypes = [P, U64, D, D, D]
+    L.uepm_server_stats_get.argtypes = [P, C.POINTER(uepm_server_stats)]
+    L.uepm_server_destroy.argtypes = [P]
     _lib = L
     return L
 

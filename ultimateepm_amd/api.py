@@ -147,6 +147,10 @@ class Plan:
                                      len(self._G), flags, nb, C.byref(self._h)))
         self.dim = lib().uepm_plan_dim(self._h)
 
+    @property
+    def handle(self):
+        return self._h
+
     def close(self):
         if self._h:
             lib().uepm_plan_destroy(self._h)
@@ -186,6 +190,29 @@ class Plan:
     def set_timing(self, on: bool):
         check(lib().uepm_plan_set_timing(self._h, int(on)))
 
+    def config(self) -> dict:
+        c = _abi.uepm_config()
+        check(lib().uepm_plan_get_config(self._h, C.byref(c)))
+        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk}
+
+    def set_config(self, tridiag: int = 0, tri_block: int = 0, bt_bands: int = 0, max_chunk: int = 0):
+        c = _abi.uepm_config(tridiag, tri_block, bt_bands, max_chunk)
+        check(lib().uepm_plan_set_config(self._h, C.byref(c)))
+
+    def tune(self, k_ptr: int, nk: int, stream: int = 0) -> dict:
+        """Autotuner: times the candidate configurations on device k-points, keeps the fastest."""
+        c = _abi.uepm_config()
+        check(lib().uepm_plan_tune(self._h, C.c_void_p(k_ptr), nk, C.c_void_p(stream), C.byref(c)))
+        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk}
+
+    def workspace_bytes(self, nk: int) -> int:
+        b = C.c_uint64(0)
+        check(lib().uepm_plan_workspace_bytes(self._h, nk, C.byref(b)))
+        return b.value
+
+    def reserve(self, nk: int):
+        check(lib().uepm_plan_reserve(self._h, nk))
+
     def last_timing(self):
         """{kernel name: (ms, launches)} of the last solve_device call."""
         out = {}
@@ -200,3 +227,75 @@ class Plan:
             out[name.value.decode()] = (ms.value, n.value)
             st += 1
         return out
+
+
+class Batch:
+    """One hipGraph over many (plan, k-set) items (uepm_batch_*)."""
+
+    def __init__(self, ctx: Context, items, branches: int = 4):
+        """items: iterable of (plan, k_ptr, nk, evals_ptr[, evecs_ptr[, grads_ptr]])."""
+        arr = (_abi.uepm_batch_item * len(items))()
+        for i, it in enumerate(items):
+            plan, kp, nk, ep = it[:4]
+            vp = it[4] if len(it) > 4 else 0
+            gp = it[5] if len(it) > 5 else 0
+            arr[i] = _abi.uepm_batch_item(plan.handle.value, kp, nk, ep, vp or None, gp or None)
+        self._h = C.c_void_p()
+        self._keep = list(items)
+        check(lib().uepm_batch_create(ctx.handle, arr, len(items), branches, C.byref(self._h)))
+
+    def launch(self, stream: int = 0):
+        check(lib().uepm_batch_launch(self._h, C.c_void_p(stream)))
+
+    def close(self):
+        if self._h:
+            lib().uepm_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request server (uepm_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, max_inflight: int = 8):
+        self._h = C.c_void_p()
+        check(lib().uepm_server_create(ctx.handle, hbm_budget_bytes, max_inflight, C.byref(self._h)))
+        self._pending = {}
+
+    def submit(self, material, nearest_neighbors: int, flags: int, nb: int, k: np.ndarray) -> int:
+        k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+        cm = material.to_c() if hasattr(material, "to_c") else material
+        req = _abi.uepm_request(C.pointer(cm), nearest_neighbors, flags, nb, _dp(k), len(k))
+        t = C.c_uint64(0)
+        check(lib().uepm_server_submit(self._h, C.byref(req), C.byref(t)))
+        self._pending[t.value] = (len(k), nb)
+        return t.value
+
+    def wait(self, ticket: int):
+        """(eigenvalues (nk, nb), latency_s, queued_s) of a submitted request."""
+        nk, nb = self._pending.pop(ticket)
+        ev = np.zeros((nk, nb))
+        lat, que = C.c_double(0), C.c_double(0)
+        check(lib().uepm_server_wait(self._h, ticket, _dp(ev), C.byref(lat), C.byref(que)))
+        return ev, lat.value, que.value
+
+    def stats(self) -> dict:
+        st = _abi.uepm_server_stats()
+        check(lib().uepm_server_stats_get(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in st._fields_}
+
+    def close(self):
+        if self._h:
+            lib().uepm_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip", "copy.hip", "tridiag_reg.hip", "backtransform.hip"]
-HOST_SOURCES = ["uepm_abi.cpp"]
+HOST_SOURCES = ["uepm_abi.cpp", "runtime.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-munsafe-fp-atomics"]
@@ -47,11 +47,11 @@ def build(verbose: bool = True) -> str:
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD_DIR, src + ".o")
-        _run([HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", *COMMON, "-c",
+        _run([HIPCC, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-pthread", *COMMON, "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, OUT)
     build_epp()
     return OUT

@@ -237,9 +237,10 @@ namespace {
 template <typename R, typename VT>
 hipError_t launch_apply(const VT* dV, int n, int ld, const VT* dtau, const double* dZ, int nb, double2* dX,
                         double* dgrads, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                        hipStream_t s) {
-    // 16 lanes per band, at most 16 bands (4 waves) per workgroup
-    const int bpw = nb >= 16 ? 16 : ((nb + 3) / 4) * 4;
+                        int bands_per_group, hipStream_t s) {
+    // 16 lanes per band, bands_per_group (4, 8 or 16) bands per workgroup
+    const int cap = (bands_per_group == 4 || bands_per_group == 8) ? bands_per_group : 16;
+    const int bpw = nb >= cap ? cap : ((nb + 3) / 4) * 4;
     const dim3 grid((unsigned)nk, (unsigned)((nb + bpw - 1) / bpw)), block(16 * bpw);
 #define UEPM_APPLY(ROWS)                                                                                       \
     hipLaunchKernelGGL((apply_q_kernel<R, VT, ROWS>), grid, block, 0, s, dV, n, ld, dtau, dZ, nb, dX, dgrads, P, \
@@ -264,14 +265,14 @@ hipError_t launch_cluster_orth(const double* dd, const double* de, const double*
 
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                              int64_t nk, hipStream_t s) {
-    return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, s);
+                              int64_t nk, int bands_per_group, hipStream_t s) {
+    return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, bands_per_group, s);
 }
 
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                              hipStream_t s) {
-    return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, s);
+                              int bands_per_group, hipStream_t s) {
+    return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, bands_per_group, s);
 }
 
 }  // namespace uepm

@@ -57,12 +57,13 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
 // reduction (FP64: over H; precision form: complex64 Vout), gradients when dgrads.
 hipError_t launch_cluster_orth(const double* dd, const double* de, const double* devals, double* dZ, int n, int nb,
                                int64_t nk, hipStream_t s);
+// bands_per_group: bands per workgroup (16 lanes each), 4, 8 or 16
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                              int64_t nk, hipStream_t s);
+                              int64_t nk, int bands_per_group, hipStream_t s);
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                              hipStream_t s);
+                              int bands_per_group, hipStream_t s);
 
 // Device-to-device copy (16-byte aligned pointers take the vector path).
 hipError_t launch_copy_bytes(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);

@@ -97,6 +97,12 @@ __device__ __forceinline__ float cross_rows4_pair(float p0x, float p0y, float p1
     return p0x + p1x;
 }
 
+// 1/x: hardware reciprocal and one Newton step (full single precision)
+__device__ __forceinline__ float rcp1(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return fmaf(fmaf(-x, r, 1.f), r, r);
+}
+
 __device__ __forceinline__ double rcp_nr(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(fma(-x, r, 1.0), r, r);
@@ -227,25 +233,24 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
             const float t = fmaf(a.x, a.x, a.y * a.y);
             part += i < NX ? t : 0.f;
         }
-        const double xn2 = (double)wave_sum(part);
+        const float xn2 = wave_sum(part);
         const f2 al = X[s + 1];
-        const double ar = al.x, ai = al.y;
-        double beta = ar, tr = 0.0, ti = 0.0, sr = 1.0, si = 0.0;
-        if (!(xn2 == 0.0 && ai == 0.0)) {
-            const double nrm = sqrt(ar * ar + ai * ai + xn2);
-            beta = ar >= 0 ? -nrm : nrm;
-            const double ib = rcp_nr(beta);
-            tr = (beta - ar) * ib;
-            ti = -ai * ib;
-            const double br = ar - beta, iden = rcp_nr(br * br + ai * ai);
+        float beta = al.x, tr = 0.f, ti = 0.f, sr = 1.f, si = 0.f;
+        if (!(xn2 == 0.f && al.y == 0.f)) {
+            const float nrm = sqrtf(fmaf(al.x, al.x, fmaf(al.y, al.y, xn2)));
+            beta = al.x >= 0.f ? -nrm : nrm;
+            const float ib = rcp1(beta);
+            tr = (beta - al.x) * ib;
+            ti = -al.y * ib;
+            const float br = al.x - beta, iden = rcp1(fmaf(br, br, al.y * al.y));
             sr = br * iden;
-            si = -ai * iden;
+            si = -al.y * iden;
         }
-        const f2 tau = mk((float)tr, (float)ti);
-        const f2 sc = mk((float)sr, (float)si);
+        const f2 tau = mk(tr, ti);
+        const f2 sc = mk(sr, si);
         if (tid == 0) {
             d[s] = 2.0 * (double)XD[s].x;
-            e[s] = beta;
+            e[s] = (double)beta;
             tau_o[s] = make_float2(tau.x, tau.y);
         }
         {
@@ -351,19 +356,23 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
             if (m >= m0) {
 #pragma unroll
                 for (int l = Lay::lmin(m); l < LR; ++l) {
-                    // delta = -(v_i conj(w_j) + w_i conj(v_j))
-                    f2 dl;
-                    dl.x = -(vr[l].x * wc[m].x + vr[l].y * wc[m].y) - (wr[l].x * vc[m].x + wr[l].y * vc[m].y);
-                    dl.y = -(vr[l].y * wc[m].x - vr[l].x * wc[m].y) - (wr[l].y * vc[m].x - wr[l].x * vc[m].y);
-                    f2& a = A[Lay::idx(l, m)];
+                    // a -= v_i conj(w_j) + w_i conj(v_j), scaled by 1, 1/2 or 0 on blocks that
+                    // straddle the diagonal
+                    f2 vi = vr[l], wi = wr[l];
                     if (m >= Lay::mcross(l)) {
                         const float h = HF[l][m - Lay::mcross(l)];
-                        a.x = fmaf(h, dl.x, a.x);
-                        a.y = fmaf(h, dl.y, a.y);
-                    } else {
-                        a.x += dl.x;
-                        a.y += dl.y;
+                        vi = mk(h * vi.x, h * vi.y);
+                        wi = mk(h * wi.x, h * wi.y);
                     }
+                    f2& a = A[Lay::idx(l, m)];
+                    a.x = fmaf(-vi.x, wc[m].x, a.x);
+                    a.x = fmaf(-vi.y, wc[m].y, a.x);
+                    a.x = fmaf(-wi.x, vc[m].x, a.x);
+                    a.x = fmaf(-wi.y, vc[m].y, a.x);
+                    a.y = fmaf(-vi.y, wc[m].x, a.y);
+                    a.y = fmaf(vi.x, wc[m].y, a.y);
+                    a.y = fmaf(-wi.y, vc[m].x, a.y);
+                    a.y = fmaf(wi.x, vc[m].y, a.y);
                 }
             }
         }

@@ -17,24 +17,18 @@
 #include <string>
 #include <vector>
 
-#include "../../include/uepm.h"
-#include "solver.hpp"
+#include "uepm_internal.hpp"
+
+namespace uepm_host {
+std::string& last_error() {
+    thread_local std::string msg;
+    return msg;
+}
+}  // namespace uepm_host
+
+using namespace uepm_host;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-    g_err = msg;
-    return code;
-}
-
-#define HIP_OK(expr)                                                                             \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail(UEPM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
-    } while (0)
 
 // src/Common/physical_constants.hpp:18-41, same operation order.
 constexpr double kPi = 3.14159265358979323846;
@@ -44,65 +38,99 @@ constexpr double kHbar = kH / (2.0 * kPi);
 constexpr double kMe = 9.1093837015e-31;
 constexpr double kBohr = 5.29177210903e-11;
 
-struct DeviceBuffer {
-    void* p = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, need);
-        if (e == hipSuccess) bytes = need;
-        return e;
-    }
-    ~DeviceBuffer() {
-        if (p) (void)hipFree(p);
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
-enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kRefine, kNumStages };
 const char* kStageNames[kNumStages] = {"assemble_kernel", "tridiag_kernel", "tri_eig_kernel", "backtransform_kernel",
                                        "refine_kernel"};
-// Extra Ritz vectors carried by the precision form beyond the nb requested,
-// so that a degenerate pair straddling band nb is refined as a whole.
-constexpr int kMixedExtra = 4;
 
 }  // namespace
 
-struct uepm_ctx {
-    int device = 0;
-    hipDeviceProp_t prop{};
-    int lds_limit = 65536;  // dynamic LDS bytes one workgroup may use
-};
+namespace uepm_host {
 
-struct uepm_plan {
-    uepm_ctx* ctx = nullptr;
-    uepm::DevParams P{};
-    unsigned flags = 0;
-    int N = 0, dim = 0, nb = 0;
-    DeviceBuffer G, H, d, e, tau, Z, W, X, S;
-    DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
-    uepm::TriPlan tp{}, tp32{};
-    bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
-    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 192)
-    bool bt_v1 = false;   // the one-wave-per-band back-transform (A/B reference)
-    int timing = 0;
-    std::vector<hipEvent_t> ev;  // pairs per stage launch
-    double stage_ms[kNumStages] = {0, 0, 0, 0, 0};
-    int64_t stage_launches[kNumStages] = {0, 0, 0, 0, 0};
-    ~uepm_plan() {
-        for (auto e : ev) (void)hipEventDestroy(e);
-    }
+int64_t plan_chunk(const uepm_plan* p, int64_t nk) {
+    const size_t mat_bytes = sizeof(double2) * (size_t)p->dim * p->dim;
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)((4ull << 30) / mat_bytes)));
+    if (p->cfg.max_chunk > 0) chunk = std::min<int64_t>(chunk, p->cfg.max_chunk);
+    return std::max<int64_t>(1, std::min(chunk, nk));
+}
+
+namespace {
+struct Sizes {
+    size_t H, d, e, tau, Vf, tauf, S32, Ev, X0, Y, S, Z, W, X;
 };
+Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
+    const int n = p->dim, nb = p->nb;
+    const bool mixed = p->flags & UEPM_MIXED;
+    const bool vec = p->flags & UEPM_VECTORS;
+    const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
+    const bool stage_vec = mixed || vec;
+    Sizes z{};
+    z.H = sizeof(double2) * (size_t)n * n * chunk;
+    z.d = z.e = sizeof(double) * (size_t)n * chunk;
+    z.tau = sizeof(double2) * (size_t)n * chunk;
+    if (mixed) {
+        z.Vf = sizeof(float2) * (size_t)n * n * chunk;
+        z.tauf = sizeof(float2) * (size_t)n * chunk;
+        if (!p->tri_reg) z.S32 = sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk;
+        z.Ev = sizeof(double) * (size_t)nbr * chunk;
+        z.X0 = sizeof(double2) * (size_t)n * nbr * chunk;
+        if (nbr > 16) z.Y = sizeof(double2) * (size_t)n * nbr * chunk;
+    } else if (!p->tri_v1) {
+        z.S = sizeof(double2) * (size_t)p->tp.scratch_elems * chunk;
+    }
+    if (stage_vec) {
+        z.Z = sizeof(double) * (size_t)n * nbr * chunk;
+        z.W = sizeof(double) * 6 * (size_t)n * nbr * chunk;
+        if (!mixed) z.X = sizeof(double2) * (size_t)n * nb * chunk;
+    }
+    return z;
+}
+}  // namespace
+
+size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk) {
+    const Sizes z = plan_sizes(p, chunk);
+    return z.H + z.d + z.e + z.tau + z.Vf + z.tauf + z.S32 + z.Ev + z.X0 + z.Y + z.S + z.Z + z.W + z.X;
+}
+
+int plan_reserve(uepm_plan* p, int64_t chunk) {
+    const Sizes z = plan_sizes(p, chunk);
+    HIP_OK(p->H.ensure(z.H));
+    HIP_OK(p->d.ensure(z.d));
+    HIP_OK(p->e.ensure(z.e));
+    HIP_OK(p->tau.ensure(z.tau));
+    if (z.Vf) HIP_OK(p->Vf.ensure(z.Vf));
+    if (z.tauf) HIP_OK(p->tauf.ensure(z.tauf));
+    if (z.S32) HIP_OK(p->S32.ensure(z.S32));
+    if (z.Ev) HIP_OK(p->Ev.ensure(z.Ev));
+    if (z.X0) HIP_OK(p->X0.ensure(z.X0));
+    if (z.Y) HIP_OK(p->Y.ensure(z.Y));
+    if (z.S) HIP_OK(p->S.ensure(z.S));
+    if (z.Z) HIP_OK(p->Z.ensure(z.Z));
+    if (z.W) HIP_OK(p->W.ensure(z.W));
+    if (z.X) HIP_OK(p->X.ensure(z.X));
+    return UEPM_OK;
+}
+
+void plan_apply_config(uepm_plan* p) {
+    const bool mixed = p->flags & UEPM_MIXED;
+    const char* reg = std::getenv("UEPM_TRIDIAG_REG");
+    const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
+    if (p->cfg.tridiag == 2) p->tri_reg = reg_ok;
+    else if (p->cfg.tridiag == 1) p->tri_reg = false;
+    else p->tri_reg = reg_ok && !(reg && reg[0] == '0');
+    int bs = p->cfg.tri_block;
+    if (bs == 0) {
+        const char* bs_env = std::getenv("UEPM_TRI_BS");
+        bs = (bs_env && std::atoi(bs_env) == 256) ? 256 : 512;
+    }
+    p->tp = uepm::tri_plan(p->dim, (int)sizeof(double2), p->ctx->lds_limit, bs);
+    p->tp32 = uepm::tri_plan(p->dim, (int)sizeof(float2), p->ctx->lds_limit, bs);
+    p->bt_bands = p->cfg.bt_bands > 0 ? p->cfg.bt_bands : 16;
+}
+
+}  // namespace uepm_host
 
 extern "C" {
 
-const char* uepm_last_error(void) { return g_err.c_str(); }
+const char* uepm_last_error(void) { return last_error().c_str(); }
 
 int uepm_ctx_create(int device, uepm_ctx** out) {
     if (!out) return fail(UEPM_ERR_INVALID_ARG, "uepm_ctx_create: null output");
@@ -346,16 +374,11 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     P.grad_pref = 2.0 * diag_factor * (P.tpa * P.tpa);
     P.frac_to_phys = 1.0 / P.tpa;
 
-    const char* bs_env = std::getenv("UEPM_TRI_BS");
-    const int tri_bs = (bs_env && std::atoi(bs_env) == 256) ? 256 : 512;
-    p->tp = uepm::tri_plan(dim, (int)sizeof(double2), ctx->lds_limit, tri_bs);
-    p->tp32 = uepm::tri_plan(dim, (int)sizeof(float2), ctx->lds_limit, tri_bs);
     const char* v1 = std::getenv("UEPM_TRIDIAG_V1");
     p->tri_v1 = v1 && v1[0] == '1';
-    const char* reg = std::getenv("UEPM_TRIDIAG_REG");
-    p->tri_reg = dim <= uepm::tridiag_reg_max_n() && !(reg && reg[0] == '0');
     const char* btv1 = std::getenv("UEPM_BT_V1");
     p->bt_v1 = btv1 && btv1[0] == '1';
+    plan_apply_config(p);
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
     if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -386,6 +409,100 @@ int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t*
     if (launches) *launches = plan->stage_launches[stage];
     if (name) *name = kStageNames[stage];
     return UEPM_OK;
+}
+
+int uepm_plan_get_config(const uepm_plan* p, uepm_config* cfg) {
+    if (!p || !cfg) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_get_config: null argument");
+    *cfg = p->cfg;
+    return UEPM_OK;
+}
+
+int uepm_plan_set_config(uepm_plan* p, const uepm_config* cfg) {
+    if (!p || !cfg) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_set_config: null argument");
+    if (cfg->tridiag < 0 || cfg->tridiag > 2)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_config.tridiag must be 0, 1 or 2");
+    if (cfg->tridiag == 2 && !((p->flags & UEPM_MIXED) && p->dim <= uepm::tridiag_reg_max_n()))
+        return fail(UEPM_ERR_INVALID_ARG, "the register-resident reduction needs UEPM_MIXED and dim <= " +
+                                              std::to_string(uepm::tridiag_reg_max_n()));
+    if (cfg->tri_block != 0 && cfg->tri_block != 256 && cfg->tri_block != 512)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_config.tri_block must be 0, 256 or 512");
+    if (cfg->bt_bands != 0 && cfg->bt_bands != 4 && cfg->bt_bands != 8 && cfg->bt_bands != 16)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_config.bt_bands must be 0, 4, 8 or 16");
+    if (cfg->max_chunk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_config.max_chunk must be >= 0");
+    p->cfg = *cfg;
+    plan_apply_config(p);
+    return UEPM_OK;
+}
+
+int uepm_plan_workspace_bytes(const uepm_plan* p, int64_t nk, uint64_t* bytes) {
+    if (!p || !bytes) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_workspace_bytes: null argument");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "nk < 0");
+    *bytes = nk == 0 ? 0 : (uint64_t)plan_workspace_bytes(p, plan_chunk(p, nk));
+    return UEPM_OK;
+}
+
+int uepm_plan_reserve(uepm_plan* p, int64_t nk) {
+    if (!p) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_reserve: null plan");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "nk < 0");
+    if (nk == 0) return UEPM_OK;
+    HIP_OK(hipSetDevice(p->ctx->device));
+    return plan_reserve(p, plan_chunk(p, nk));
+}
+
+int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, uepm_config* chosen) {
+    if (!p || !dk || nk <= 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_tune: null plan, no k-points");
+    HIP_OK(hipSetDevice(p->ctx->device));
+    hipStream_t s = (hipStream_t)stream_;
+    // candidates: reduction variant x workgroup size, back-transform band groups
+    std::vector<uepm_config> cand;
+    const bool reg_ok = (p->flags & UEPM_MIXED) && p->dim <= uepm::tridiag_reg_max_n();
+    for (int bt : {8, 16}) {
+        if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk});
+        for (int tb : {256, 512}) cand.push_back({1, tb, bt, p->cfg.max_chunk});
+    }
+    const uepm_config saved = p->cfg;
+    const int saved_timing = p->timing;
+    p->timing = 0;
+    DeviceBuffer ev, vec, gr;
+    int rc = UEPM_OK;
+    if (ev.ensure(sizeof(double) * (size_t)nk * p->nb) != hipSuccess)
+        rc = fail(UEPM_ERR_NO_MEMORY, "uepm_plan_tune: out of device memory");
+    const bool want_vec = p->flags & UEPM_VECTORS;
+    if (rc == UEPM_OK && want_vec &&
+        vec.ensure(sizeof(double2) * (size_t)nk * p->nb * p->dim) != hipSuccess)
+        rc = fail(UEPM_ERR_NO_MEMORY, "uepm_plan_tune: out of device memory");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == UEPM_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = fail(UEPM_ERR_DEVICE, "uepm_plan_tune: hipEventCreate failed");
+    float best = 3.4e38f;
+    uepm_config best_cfg = saved;
+    for (size_t c = 0; rc == UEPM_OK && c < cand.size(); ++c) {
+        p->cfg = cand[c];
+        plan_apply_config(p);
+        double* dv = want_vec ? vec.as<double>() : nullptr;
+        rc = uepm_solve_device(p, dk, nk, ev.as<double>(), dv, nullptr, s);  // warm-up (and workspace)
+        if (rc != UEPM_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(UEPM_ERR_DEVICE, "hipEventRecord"); break; }
+        for (int r = 0; r < 2 && rc == UEPM_OK; ++r) rc = uepm_solve_device(p, dk, nk, ev.as<double>(), dv, nullptr, s);
+        if (rc != UEPM_OK) break;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(UEPM_ERR_DEVICE, "uepm_plan_tune: event timing failed");
+            break;
+        }
+        if (ms < best) {
+            best = ms;
+            best_cfg = cand[c];
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    p->timing = saved_timing;
+    p->cfg = rc == UEPM_OK ? best_cfg : saved;
+    plan_apply_config(p);
+    if (rc == UEPM_OK && chosen) *chosen = p->cfg;
+    return rc;
 }
 
 int uepm_assemble_device(uepm_plan* p, const double* dk, int64_t nk, double* dH, void* stream) {
@@ -420,28 +537,10 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     const bool mixed = p->flags & UEPM_MIXED;
     const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
     const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
-    // Batch size: the full-matrix workspace stays under ~4 GiB.
-    const size_t mat_bytes = sizeof(double2) * (size_t)n * n;
-    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(4096, (int64_t)((4ull << 30) / mat_bytes)));
-    chunk = std::min(chunk, nk);
-    HIP_OK(p->H.ensure(mat_bytes * chunk));
-    HIP_OK(p->d.ensure(sizeof(double) * n * chunk));
-    HIP_OK(p->e.ensure(sizeof(double) * n * chunk));
-    HIP_OK(p->tau.ensure(sizeof(double2) * n * chunk));
-    if (mixed) {
-        HIP_OK(p->Vf.ensure(sizeof(float2) * (size_t)n * n * chunk));
-        HIP_OK(p->tauf.ensure(sizeof(float2) * (size_t)n * chunk));
-        if (!p->tri_reg) HIP_OK(p->S32.ensure(sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk));
-        HIP_OK(p->Ev.ensure(sizeof(double) * (size_t)nbr * chunk));
-        HIP_OK(p->X0.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
-        HIP_OK(p->Y.ensure(sizeof(double2) * (size_t)n * nbr * chunk));
-    } else if (!p->tri_v1) {
-        HIP_OK(p->S.ensure(sizeof(double2) * (size_t)p->tp.scratch_elems * chunk));
-    }
-    if (stage_vec) {
-        HIP_OK(p->Z.ensure(sizeof(double) * (size_t)n * nbr * chunk));
-        HIP_OK(p->W.ensure(sizeof(double) * 6 * (size_t)n * nbr * chunk));
-        if (!mixed && !devecs) HIP_OK(p->X.ensure(sizeof(double2) * (size_t)n * nb * chunk));
+    const int64_t chunk = plan_chunk(p, nk);
+    {
+        const int st = plan_reserve(p, chunk);
+        if (st != UEPM_OK) return st;
     }
     const int nchunks = (int)((nk + chunk - 1) / chunk);
     if (p->timing) {
@@ -490,7 +589,8 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
                 HIP_OK(uepm::launch_cluster_orth(p->d.as<double>(), p->e.as<double>(), p->Ev.as<double>(),
                                                  p->Z.as<double>(), n, nbr, c, s));
                 HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, p->tauf.as<float2>(), p->Z.as<double>(), nbr,
-                                                p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, s));
+                                                p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
+                                                s));
             }
             HIP_OK(rec(ci, kBack, 1));
             HIP_OK(rec(ci, kRefine, 0));
@@ -526,7 +626,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
                                                  p->Z.as<double>(), n, nb, c, s));
                 HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, p->tau.as<double2>(), p->Z.as<double>(), nb, Xc,
                                                 dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                p->G.as<int32_t>(), c, s));
+                                                p->G.as<int32_t>(), c, p->bt_bands, s));
             }
             HIP_OK(rec(ci, kBack, 1));
         }

@@ -1,0 +1,98 @@
+// uepm_internal.hpp — definitions shared by the host-side sources of the C ABI
+// (uepm_abi.cpp: plans and solves; runtime.cpp: tuning, batch graphs, serving).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/uepm.h"
+#include "solver.hpp"
+
+namespace uepm_host {
+
+// per-thread message behind uepm_last_error()
+std::string& last_error();
+
+inline int fail(int code, const std::string& msg) {
+    last_error() = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                                       \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return ::uepm_host::fail(UEPM_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ~DeviceBuffer() { release(); }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kRefine, kNumStages };
+// Extra Ritz vectors carried by the precision form beyond the nb requested,
+// so that a degenerate pair straddling band nb is refined as a whole.
+constexpr int kMixedExtra = 4;
+
+}  // namespace uepm_host
+
+struct uepm_ctx {
+    int device = 0;
+    hipDeviceProp_t prop{};
+    int lds_limit = 65536;  // dynamic LDS bytes one workgroup may use
+};
+
+struct uepm_plan {
+    uepm_ctx* ctx = nullptr;
+    uepm::DevParams P{};
+    unsigned flags = 0;
+    int N = 0, dim = 0, nb = 0;
+    uepm_host::DeviceBuffer G, H, d, e, tau, Z, W, X, S;
+    uepm_host::DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
+    uepm::TriPlan tp{}, tp32{};
+    uepm_config cfg{};    // requested configuration (0 fields = automatic)
+    bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
+    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 192)
+    bool bt_v1 = false;   // the one-wave-per-band back-transform (A/B reference)
+    int bt_bands = 16;    // bands per back-transform workgroup
+    int timing = 0;
+    std::vector<hipEvent_t> ev;  // pairs per stage launch
+    double stage_ms[uepm_host::kNumStages] = {0, 0, 0, 0, 0};
+    int64_t stage_launches[uepm_host::kNumStages] = {0, 0, 0, 0, 0};
+    ~uepm_plan() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+namespace uepm_host {
+// k-points per launch for a solve of nk points under the plan's configuration
+int64_t plan_chunk(const uepm_plan* p, int64_t nk);
+// workspace of a chunk (bytes) and its allocation; `vectors` reserves the
+// eigenvector scratch used when gradients are asked without eigenvectors
+size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk);
+int plan_reserve(uepm_plan* p, int64_t chunk);
+// applies p->cfg to the derived launch parameters (validated by the caller)
+void plan_apply_config(uepm_plan* p);
+}  // namespace uepm_host
