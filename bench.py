@@ -104,6 +104,94 @@ def copy_calibration(mib: int, reps: int) -> int:
     return 0
 
 
+def serving_leg(ctx, requests: int, seed: int = SEED, budget_gib: int = 64, inflight: int = 8) -> dict:
+    """Serving form: Poisson arrivals at 200 req/s (seeded); material uniform over
+    Si/Ge/GaAs/InP, n in 6..14 shells, SOC on for half (off for InP), log-uniform
+    1..4096 k-points, 8..24 bands; the same request stream in FP64 and in the
+    precision form on a continuous-batching server (HBM budget, in-flight slots)."""
+    from ultimateepm_amd import api, serving
+
+    wl = serving.poisson_workload(seed, requests, rate=200.0, n_min=6, n_max=14, k_max=4096)
+    srv = api.Server(ctx, budget_gib << 30, max_inflight=inflight)
+    out = {"requests": requests, "rate_req_s": 200.0, "hbm_budget_gib": budget_gib, "max_inflight": inflight,
+           "workload": "Poisson 200 req/s (seed %d); {Si,Ge,GaAs,InP} x n 6..14 x SOC half; k log-uniform "
+                       "1..4096; nb 8..24; non-local" % seed}
+    res = {}
+    for mode in ("fp64", "mixed"):
+        serving.run(srv, wl[: min(8, len(wl))], mixed=(mode == "mixed"), realtime=False)  # plan warm-up
+        r, summ = serving.run(srv, wl, mixed=(mode == "mixed"), realtime=True)
+        res[mode] = r
+        out[mode] = summ
+    worst = max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(res["fp64"], res["mixed"]))
+    out["mixed_vs_fp64_max_meV"] = worst * 1e3
+    srv.close()
+    return out
+
+
+def batching_leg(ctx, nk_per_item: int, branches: int = 8, ws_cap_gib: float = 2.0, seed: int = SEED) -> dict:
+    """Batching form: {Si, Ge, GaAs, InP} x n in {6, 8, 10, 12} shells x SOC off/on
+    (InP + SOC excluded: its anion period has no spin-orbit radial model), nk
+    seeded first-BZ k-points each, all captured into one hipGraph (concurrent
+    branches) and launched once; reduction variant and band grouping chosen per
+    dimension by the autotuner.  Run in the precision form and in FP64."""
+    from ultimateepm_amd import api
+    from ultimateepm_amd.material import load
+
+    combos = [(m, n, soc) for m in ("Si", "Ge", "GaAs", "InP") for n in (6, 8, 10, 12) for soc in (False, True)
+              if not (m == "InP" and soc)]
+    ks = {}
+    out = {"items": len(combos), "nk_per_item": nk_per_item, "branches": branches}
+    evals = {}
+    for mode in ("mixed", "fp64"):
+        tuned = {}  # dimension bucket -> configuration chosen by the autotuner
+        plans, bufs, items = [], [], []
+        t_tune = time.perf_counter()
+        for i, (m, n, soc) in enumerate(combos):
+            nb = 16 if soc else 8
+            flags = api.NONLOCAL | (api.SOC if soc else 0) | (api.MIXED if mode == "mixed" else 0)
+            p = api.Plan(ctx, load(m), api.basis_vectors(n), flags, nb)
+            per_k = max(1, p.workspace_bytes(1))
+            cap = int(max(1, min(4096, ws_cap_gib * (1 << 30) // per_k)))
+            key = (p.dim, mode)
+            if key not in tuned:
+                sample = api.bz_sample(seed + 7, min(1024, nk_per_item))
+                dsample = api.DeviceArray.from_host(ctx, sample)
+                p.set_config(max_chunk=cap)
+                tuned[key] = p.tune(dsample.ptr, len(sample))
+                dsample.free()
+            cfg = dict(tuned[key])
+            cfg["max_chunk"] = cap
+            p.set_config(**cfg)
+            if i not in ks:
+                ks[i] = api.bz_sample(seed + 1000 + i, nk_per_item)
+            dk = api.DeviceArray.from_host(ctx, ks[i])
+            de = api.DeviceArray(ctx, (nk_per_item, nb))
+            plans.append(p)
+            bufs.append((dk, de))
+            items.append((p, dk.ptr, nk_per_item, de.ptr))
+        t_tune = time.perf_counter() - t_tune
+        b = api.Batch(ctx, items, branches=branches)
+        b.launch()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        b.launch()
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        evals[mode] = [de.numpy() for _, de in bufs]
+        out[mode] = {"kpts_per_s": round(len(combos) * nk_per_item / dt, 1), "graph_launch_s": round(dt, 4),
+                     "tune_and_setup_s": round(t_tune, 2),
+                     "tuned": {f"dim{d}": c for (d, _), c in sorted(tuned.items())}}
+        b.close()
+        for p in plans:
+            p.close()
+        for dk, de in bufs:
+            dk.free()
+            de.free()
+    out["mixed_over_fp64"] = round(out["mixed"]["kpts_per_s"] / out["fp64"]["kpts_per_s"], 3)
+    out["mixed_vs_fp64_max_meV"] = max(float(np.abs(a - b).max()) for a, b in zip(evals["mixed"], evals["fp64"])) * 1e3
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +203,12 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
+    ap.add_argument("--form", default="throughput", choices=["throughput", "serving", "batching"],
+                    help="throughput: the headline batched solve (default); serving / batching: those forms alone")
+    ap.add_argument("--requests", type=int, default=200, help="requests of --form serving")
+    ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
+    ap.add_argument("--serving-requests", type=int, default=100,
+                    help="requests of the serving leg of the default run (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -129,6 +223,29 @@ def main() -> int:
 
     from ultimateepm_amd import api
     from ultimateepm_amd.material import load
+
+    if args.form == "batching":
+        ctx = api.Context(0)
+        bt = batching_leg(ctx, args.batch_nk)
+        line = {"metric": "k-pts/s batching form: 28 (material, cutoff, SOC) plans in one graph launch",
+                "value": bt["mixed"]["kpts_per_s"], "unit": "k-pts/s", "n_gpus": 1, "higher_is_better": True,
+                "data": "synthetic", "dtype": "c64+f64",
+                "config": {"workload": "{Si,Ge,GaAs,InP} x n{6,8,10,12} x SOC{off,on} (no InP SOC), "
+                                       f"{args.batch_nk} seeded BZ k-points each, nb 8 (16 with SOC)",
+                           "form": "batching"},
+                "batching": bt}
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return 0
+    if args.form == "serving":
+        ctx = api.Context(0)
+        sv = serving_leg(ctx, args.requests)
+        line = {"metric": "serving p50/p99 latency (precision form), Poisson 200 req/s", "value": sv["mixed"]["latency_p99_ms"],
+                "unit": "ms p99", "n_gpus": 1, "higher_is_better": False, "data": "synthetic",
+                "dtype": "c64+f64", "config": {"workload": sv.pop("workload"), "form": "serving"}, "serving": sv}
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return 0
 
     mat = load("Si")
     G = api.basis_vectors(args.shells)
@@ -236,6 +353,15 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    if args.serving_requests > 0:
+        sv = serving_leg(ctx, args.serving_requests)
+        out["serving"] = {
+            "workload": sv["workload"], "requests": sv["requests"],
+            "mixed_p50_ms": sv["mixed"]["latency_p50_ms"], "mixed_p99_ms": sv["mixed"]["latency_p99_ms"],
+            "fp64_p50_ms": sv["fp64"]["latency_p50_ms"], "fp64_p99_ms": sv["fp64"]["latency_p99_ms"],
+            "mixed_kpts_per_s": sv["mixed"]["kpts_per_s"], "fp64_kpts_per_s": sv["fp64"]["kpts_per_s"],
+            "mixed_vs_fp64_max_meV": sv["mixed_vs_fp64_max_meV"],
+            "hbm_budget_gib": sv["hbm_budget_gib"], "max_inflight": sv["max_inflight"]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)
