@@ -295,7 +295,7 @@ def main() -> int:
     stage_ms = r["stage_ms"]
     dominant = max(stage_ms, key=lambda k: stage_ms[k][0])
     dom_ms, dom_launches = stage_ms[dominant]
-    if dominant == "tridiag_kernel":
+    if dominant.startswith("tridiag"):
         flops = tridiag_flops(r["dim"]) * args.nk * args.steps
         peak = FP32_MFMA_PEAK_TFLOPS if head == "mixed" else FP64_PEAK_TFLOPS
     else:

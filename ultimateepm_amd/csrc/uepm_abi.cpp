@@ -407,7 +407,7 @@ int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t*
     if (stage < 0 || stage >= kNumStages) return fail(UEPM_ERR_OUT_OF_RANGE, "no such stage");
     if (ms) *ms = plan->stage_ms[stage];
     if (launches) *launches = plan->stage_launches[stage];
-    if (name) *name = kStageNames[stage];
+    if (name) *name = plan->stage_name[stage][0] ? plan->stage_name[stage] : kStageNames[stage];
     return UEPM_OK;
 }
 
@@ -556,6 +556,12 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end], s);
     };
     bool used[kNumStages] = {true, true, true, stage_vec, mixed};
+    p->stage_name[kTridiag] = mixed ? (p->tri_reg ? "tridiag_reg_kernel<c64>" : "tridiag_blocked_kernel<c64>")
+                                    : (p->tri_v1 ? "tridiag_kernel<c128>" : "tridiag_blocked_kernel<c128>");
+    p->stage_name[kTriEig] = "tri_eig_wave_kernel";
+    p->stage_name[kBack] = p->bt_v1 ? "backtransform_kernel"
+                                    : (mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>");
+    p->stage_name[kRefine] = nbr <= 16 ? "refine_mfma_kernel" : "refine_kernel";
     for (int ci = 0; ci < nchunks; ++ci) {
         const int64_t c0 = (int64_t)ci * chunk;
         const int64_t c = std::min(chunk, nk - c0);

@@ -79,6 +79,7 @@ struct uepm_plan {
     int bt_bands = 16;    // bands per back-transform workgroup
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
+    const char* stage_name[uepm_host::kNumStages] = {"assemble_kernel", "", "", "", ""};  // kernels of the last solve
     double stage_ms[uepm_host::kNumStages] = {0, 0, 0, 0, 0};
     int64_t stage_launches[uepm_host::kNumStages] = {0, 0, 0, 0, 0};
     ~uepm_plan() {
