@@ -192,6 +192,98 @@ def batching_leg(ctx, nk_per_item: int, branches: int = 8, ws_cap_gib: float = 2
     return out
 
 
+def mesh_leg(ctx, steps: int, n_energies: int = 1000, cpu: bool = True, cpu_sample: int = 512) -> dict:
+    """BandsOnBZ + DOS_MeshBZ on the reference's example mesh (examples/meshes/
+    bz_mesh.msh: 14,705 vertices, 73,728 tetrahedra): Si local Cohen-Bergstresser
+    factors, n = 10 shells (N = 137), 4 + 12 + 4 levels with Hellmann-Feynman
+    gradients at every vertex (device-resident k), then the tetrahedron DOS of
+    the 16 exported bands on n_energies energies each."""
+    import gzip
+    import shutil
+    import tempfile
+
+    from ultimateepm_amd import api, mesh
+    from ultimateepm_amd.material import PARAMS_DIR, load
+
+    mat = load("Si", os.path.join(PARAMS_DIR, "materials-local-cohen.yaml"))
+    tmp = tempfile.mkdtemp(prefix="uepm_mesh_")
+    path = os.path.join(tmp, "bz_mesh.msh")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "bz_mesh.msh.gz"), "rb") as src, open(path, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    m = mesh.MeshBZ.read_mesh_geometry_from_msh_file(ctx, path, mat.get_lattice_constant_meter())
+    shutil.rmtree(tmp, ignore_errors=True)
+    nv, nt, _ = m.counts()
+    G = api.basis_vectors(10)
+    nb = 20
+    k = m.kpoints(False)
+    dk = api.DeviceArray.from_host(ctx, k)
+    stream = ctx.stream()
+    out = {"workload": f"Si local (materials-local-cohen.yaml), n=10 (N={len(G)}), {nb} levels + gradients at "
+                       f"the {nv} vertices of bz_mesh.msh; tetrahedron DOS of 16 bands x {n_energies} energies "
+                       f"over its {nt} tetrahedra", "vertices": nv, "tetrahedra": nt}
+    evals = {}
+    for mode in ("fp64", "mixed"):
+        flags = api.GRADIENTS | (api.MIXED if mode == "mixed" else 0)
+        plan = api.Plan(ctx, mat, G, flags, nb)
+        dE, dG = api.DeviceArray(ctx, (nv, nb)), api.DeviceArray(ctx, (nv, nb, 3))
+        plan.solve_device(dk.ptr, nv, dE.ptr, 0, dG.ptr, stream.handle)
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.solve_device(dk.ptr, nv, dE.ptr, 0, dG.ptr, stream.handle)
+        stream.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        out[f"{mode}_bands_kpts_per_s"] = round(nv / dt, 1)
+        out[f"{mode}_bands_ms"] = round(dt * 1e3, 3)
+        evals[mode] = (dE, dG)
+        plan.close()
+    E64 = evals["fp64"][0].numpy()
+    out["mixed_vs_fp64_max_meV"] = float(np.abs(evals["mixed"][0].numpy() - E64).max() * 1e3)
+    # DOS of the 16 bands BandsOnBZ exports (4 valence, 12 conduction), on the device bands
+    dE, dG = evals["fp64"]
+    m.set_bands_device(dE.ptr, dG.ptr, nb, 0, 16, stream.handle)
+    lo, hi = E64[:, :16].min(axis=0) - 0.1, E64[:, :16].max(axis=0) + 0.1
+    den, ddos = api.DeviceArray(ctx, (16, n_energies)), api.DeviceArray(ctx, (16, n_energies))
+    row = n_energies * 8
+
+    def dos_all():
+        for b in range(16):
+            m.dos_device(b, n_energies, float(lo[b]), float(hi[b]), 1.0, 2.0, False, den.ptr + b * row,
+                         ddos.ptr + b * row, stream.handle)
+
+    dos_all()
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dos_all()
+    stream.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    out["dos_ms_16_bands"] = round(dt * 1e3, 3)
+    out["dos_tetra_energy_pairs_per_s"] = round(16 * n_energies * nt / dt, 1)
+    if cpu:
+        from oracle import oracle
+
+        sel = np.arange(0, nv, max(1, nv // cpu_sample))[:cpu_sample]
+        t0 = time.perf_counter()
+        oracle.solve(mat, G, api.GRADIENTS, nb, k[sel], nthreads=16)
+        cpu_k = len(sel) / (time.perf_counter() - t0)
+        geo = m.geometry()
+        Eh, Gh = dE.numpy()[:, :16], dG.numpy()[:, :16]
+        t0 = time.perf_counter()
+        oracle.tetra_dos(geo["xyz"], geo["tetrahedra"], Eh, Gh, 4, n_energies, float(lo[4]), float(hi[4]), 1.0, 2.0,
+                         nthreads=16)
+        cpu_dos = time.perf_counter() - t0
+        out["cpu_baseline"] = {"bands_kpts_per_s": round(cpu_k, 1), "dos_ms_per_band": round(cpu_dos * 1e3, 1),
+                               "cores": 16, "kind": "port",
+                               "sample": f"oracle solve at {len(sel)} mesh vertices (FP64, gradients); oracle "
+                                         f"tetrahedron DOS of band 4 on {n_energies} energies"}
+        out["gpu_over_cpu_bands"] = round(out["fp64_bands_kpts_per_s"] / cpu_k, 1)
+        out["gpu_over_cpu_dos"] = round(cpu_dos * 1e3 / (out["dos_ms_16_bands"] / 16), 1)
+    stream.close()
+    m.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +301,8 @@ def main() -> int:
     ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving leg of the default run (0: skip)")
+    ap.add_argument("--mesh-steps", type=int, default=3,
+                    help="timed repetitions of the mesh leg of the default run (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -362,6 +456,8 @@ def main() -> int:
             "mixed_kpts_per_s": sv["mixed"]["kpts_per_s"], "fp64_kpts_per_s": sv["fp64"]["kpts_per_s"],
             "mixed_vs_fp64_max_meV": sv["mixed_vs_fp64_max_meV"],
             "hbm_budget_gib": sv["hbm_budget_gib"], "max_inflight": sv["max_inflight"]}
+    if args.mesh_steps > 0:
+        out["mesh"] = mesh_leg(ctx, args.mesh_steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -226,6 +226,137 @@ int uepm_server_stats_get(uepm_server* server, uepm_server_stats* stats);
 /* Drains the queue and the running requests, then frees everything. */
 int uepm_server_destroy(uepm_server* server);
 
+/* ---- Brillouin-zone mesh files (gmsh MSH 4.1, ASCII) ------------------------
+ * Replaces the gmsh calls of MeshBZ::read_mesh_geometry_from_msh_file
+ * (src/BZ_MESH/bz_mesh.cpp:67-191: getNodes, getElements of dimension 3),
+ * MeshBZ::read_mesh_bands_from_msh_file (:331-425: the NodeData views) and the
+ * view writes of MeshBZ::export_selected_bands_to_gmsh (:1426-1545).  Host-only.
+ * Vertex index = position of the node in the file; tetrahedra come back as
+ * 0-based vertex indices.  Binary MSH, MSH 2 and volume elements other than
+ * 4-node tetrahedra -> UEPM_ERR_INVALID_ARG; a malformed file -> UEPM_ERR_RUNTIME
+ * (the reference's runtime_error). */
+typedef struct uepm_msh uepm_msh;
+int uepm_msh_read(const char* path, uepm_msh** out);
+int uepm_msh_counts(const uepm_msh* msh, int64_t* n_nodes, int64_t* n_tetra, int32_t* n_views);
+/* tags: n_nodes (may be NULL); xyz: n_nodes x 3 as written in the file (may be NULL) */
+int uepm_msh_nodes(const uepm_msh* msh, uint64_t* tags, double* xyz);
+int uepm_msh_tetrahedra(const uepm_msh* msh, int32_t* vertex_index /* n_tetra x 4 */);
+/* Views in file order: name (NUL-terminated, truncated to name_capacity) and
+ * components per node (1: band energies, 3: gradients); data is
+ * n_nodes x ncomp in vertex order, NaN where the view has no value. */
+int uepm_msh_view_info(const uepm_msh* msh, int32_t view, char* name, int32_t name_capacity, int32_t* ncomp);
+int uepm_msh_view_data(const uepm_msh* msh, int32_t view, double* data);
+void uepm_msh_free(uepm_msh* msh);
+/* Appends one NodeData view (`ncomp` values per node, ASCII, %.17g) to
+ * out_path.  When out_path does not exist it is first created with the mesh
+ * sections of mesh_path (its data views left out), as gmsh's
+ * PostProcessing.SaveMesh does for the reference's first view. */
+int uepm_msh_append_node_view(const char* out_path, const char* mesh_path, const char* name, int32_t ncomp,
+                              const uint64_t* tags, int64_t n, const double* data);
+
+/* ---- Brillouin-zone mesh driver and tetrahedron DOS -------------------------
+ * Replaces MeshBZ (src/BZ_MESH/bz_mesh.hpp) on the band-solve side: the mesh
+ * of the zone, the bands at its vertices (BandsOnBZ), the k-star fan-out from
+ * the irreducible wedge, band views in and out, and the density of states
+ * (DOS_MeshBZ).  Positions are held as MeshBZ holds them after reading (1/m
+ * when normalised by 2 pi / a, bz_mesh.cpp:90-105); energies in eV, gradients
+ * in eV m.  Vertex coordinates, bands and the DOS workspace go to the device
+ * at the first device call.  ctx may be NULL for a host-only mesh (reading,
+ * k-star maps, fan-out, files); its device calls -> UEPM_ERR_INVALID_ARG.
+ * Calls on one mesh are not thread-safe (as a plan). */
+typedef struct uepm_bzmesh uepm_bzmesh;
+/* From arrays: xyz nv x 3 (1/m); tet nt x 4 vertex indices in [0, nv);
+ * tetra_in_wedge nt flags or NULL.  No lattice constant: only the explicit-
+ * range DOS (uepm_bzmesh_dos) and band setters apply. */
+int uepm_bzmesh_create(uepm_ctx* ctx, const double* xyz, int64_t nv, const int32_t* tet, int64_t nt,
+                       const uint8_t* tetra_in_wedge, uepm_bzmesh** out);
+/* MeshBZ::read_mesh_geometry_from_msh_file (bz_mesh.cpp:67-191) without its
+ * final k-star load: positions x 2 pi / a (normalize != 0), degenerate
+ * tetrahedra (|V| < 1e-12) dropped, wedge flag per tetrahedron from its
+ * barycenter, total volume and reduce-BZ factor V / ((2 pi)^3 / (a^3 / 4)). */
+int uepm_bzmesh_read_msh(uepm_ctx* ctx, const char* path, double lattice_constant_m, int normalize_by_fourier_factor,
+                         uepm_bzmesh** out);
+int uepm_bzmesh_destroy(uepm_bzmesh* mesh);
+int uepm_bzmesh_counts(const uepm_bzmesh* mesh, int64_t* n_vertices, int64_t* n_tetra, int32_t* n_bands);
+/* Any output may be NULL: xyz nv x 3, tet nt x 4, wedge flags nt, node tags nv. */
+int uepm_bzmesh_geometry(const uepm_bzmesh* mesh, double* xyz, int32_t* tet, uint8_t* tetra_in_wedge, uint64_t* tags,
+                         double* total_volume, double* reduce_bz_factor);
+/* MeshBZ::load_kstar_ibz_to_bz (bz_mesh.cpp:198-272); path NULL reads
+ * "<mesh stem>_kstar_ibz_to_bz.txt" in the working directory, as the
+ * reference.  Missing file, a row whose vertex is outside the wedge, orbit
+ * sizes not adding up to the vertex count or an index out of range ->
+ * UEPM_ERR_RUNTIME. */
+int uepm_bzmesh_load_kstar(uepm_bzmesh* mesh, const char* path);
+/* Not in MeshBZ: writes that file for a mesh closed under O_h (the format of
+ * apps/bz_meshing.cpp:644-657): every wedge vertex with its orbit under the
+ * 48 signed permutations.  An orbit point that is not a vertex ->
+ * UEPM_ERR_RUNTIME. */
+int uepm_bzmesh_write_kstar(const uepm_bzmesh* mesh, const char* path);
+/* Wedge vertices in k-star file order (ids may be NULL to query count). */
+int uepm_bzmesh_wedge_vertices(const uepm_bzmesh* mesh, int64_t* ids, int64_t* count);
+/* k-points of compute_band_structure_over_mesh (bz_mesh.cpp:1236-1252),
+ * reduced units: the wedge vertices (use_iwedge) or all; k may be NULL. */
+int uepm_bzmesh_kpoints(const uepm_bzmesh* mesh, int use_iwedge, double* k, int64_t* count);
+/* Adjusted E (count x nb) and G (count x nb x 3) at those k-points -> the
+ * vertices, the wedge fan-out with gradients mapped by the signed permutation
+ * between the two points (distribute_energies_from_iw_wedge_to_full_bz,
+ * bz_mesh.cpp:1383-1424, apply_same_symmetry_operation :1314-1381), band
+ * extrema, valence = minimum below 1e-6 eV (:1280-1301). */
+int uepm_bzmesh_set_mesh_bands(uepm_bzmesh* mesh, int use_iwedge, const double* E, const double* G, int32_t nb);
+/* compute_band_structure_over_mesh (bz_mesh.cpp:1230-1303) in one call: solve
+ * on the GPU with a plan carrying UEPM_GRADIENTS, uepm_adjust_values (no
+ * conduction shift), uepm_bzmesh_set_mesh_bands.  band_gap may be NULL. */
+int uepm_bzmesh_compute_bands(uepm_bzmesh* mesh, uepm_plan* plan, int use_iwedge, double* band_gap);
+/* Host copies, vertex-major: E nv x nb, G nv x nb x 3 (either may be NULL). */
+int uepm_bzmesh_bands(const uepm_bzmesh* mesh, double* E, double* G, int32_t* nb);
+int uepm_bzmesh_band_info(const uepm_bzmesh* mesh, int32_t band, int32_t* is_conduction, int64_t* local_index,
+                          double* min_energy, double* max_energy);
+int uepm_bzmesh_band_groups(const uepm_bzmesh* mesh, int64_t* n_valence, int64_t* valence_start,
+                            int64_t* n_conduction, int64_t* conduction_start);
+/* Host E (nv x nb) and G (nv x nb x 3), vertex-major; finite; both required.
+ * Band info as after a solve. */
+int uepm_bzmesh_set_bands(uepm_bzmesh* mesh, const double* E, const double* G, int32_t nb);
+/* Device E (nv x ld) and G (nv x ld x 3) as uepm_solve_device writes them,
+ * bands [first, first + nb); asynchronous on stream.  Leaves no host copy:
+ * only the explicit-range DOS applies afterwards. */
+int uepm_bzmesh_set_bands_device(uepm_bzmesh* mesh, const double* d_E, const double* d_G, int32_t ld, int32_t first,
+                                 int32_t nb, void* stream);
+/* MeshBZ::compute_dos_at_energy_and_band (bz_mesh.cpp:844-862, use_interp =
+ * false) at e_min + i (e_max - e_min)/(n_points - 1), i < n_points: per
+ * tetrahedron Tetra::compute_tetra_dos_energy_band (mesh_tetra.cpp:477-525),
+ * summed, x reduce_bz_factor, x spin_degeneracy, and with wedge_only over the
+ * wedge tetrahedra only, x 48.  States / (eV m^3); energies may be NULL. */
+int uepm_bzmesh_dos(uepm_bzmesh* mesh, int32_t band, int32_t n_points, double e_min, double e_max,
+                    double reduce_bz_factor, double spin_degeneracy, int wedge_only, double* energies, double* dos);
+int uepm_bzmesh_dos_device(uepm_bzmesh* mesh, int32_t band, int32_t n_points, double e_min, double e_max,
+                           double reduce_bz_factor, double spin_degeneracy, int wedge_only, double* d_energies,
+                           double* d_dos, void* stream);
+/* MeshBZ::compute_dos_band_at_band_auto (bz_mesh.cpp:889-916): range = band
+ * extrema -/+ 0.1 eV, the mesh's reduce-BZ factor, spin degeneracy 2.
+ * use_interp reproduces the reference's stub (all zeros). */
+int uepm_bzmesh_dos_auto(uepm_bzmesh* mesh, int32_t band, int32_t n_points, int use_interp, int use_iw,
+                         double* energies, double* dos);
+/* MeshBZ::export_selected_bands_to_gmsh (bz_mesh.cpp:1426-1545): ASCII views
+ * "band_<i>" / "grad_band_<i>" appended through uepm_msh_append_node_view
+ * (model_msh_path supplies the mesh when out_path is new). */
+int uepm_bzmesh_export_msh(const uepm_bzmesh* mesh, const char* out_path, int64_t n_valence, int64_t n_conduction,
+                           int highest_valence_as_band0, const char* model_msh_path, int write_gradients);
+/* MeshBZ::export_energies_and_gradients_to_vtk (bz_mesh.cpp:1647-1673). */
+int uepm_bzmesh_export_vtk(const uepm_bzmesh* mesh, const char* path);
+/* MeshBZ::read_mesh_bands_from_msh_file (bz_mesh.cpp:331-425), n < 0 keeps
+ * every band of that kind. */
+int uepm_bzmesh_read_bands_msh(uepm_bzmesh* mesh, const char* path, int32_t n_conduction, int32_t n_valence,
+                               int auto_shift_conduction_band, int set_positive_valence_band);
+/* MeshBZ::apply_scissor (bz_mesh.cpp:444-461). */
+int uepm_bzmesh_apply_scissor(uepm_bzmesh* mesh, double value);
+
+/* BandStructure::AdjustValues (src/EPP/BandStructure.cpp:203-262) over
+ * E (nk x nb) in place: the first level pair from level 2 up whose gap
+ * exceeds 0.35 eV sets the valence maximum (fallback: level 3), which is
+ * subtracted (the conduction minimum from levels >= 4, or 8 with soc, when
+ * min_cond_to_zero).  band_gap may be NULL.  Host-only. */
+int uepm_adjust_values(double* E, int64_t nk, int32_t nb, int soc, int min_cond_to_zero, double* band_gap);
+
 /* ---- device plumbing ------------------------------------------------------
  * Device memory, streams and synchronisation on the context's device, for
  * callers without a GPU runtime of their own (scripts, tests, FFI hosts).

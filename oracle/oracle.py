@@ -64,6 +64,10 @@ def lib() -> C.CDLL:
         L.orc_B_radial.restype = C.c_double
         L.orc_B_radial.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_soc_block.argtypes = [P, D, D, D, D, D]
+        U8 = C.POINTER(C.c_uint8)
+        L.orc_tetra_dos.argtypes = [D, C.c_int64, I32, C.c_int64, U8, D, D, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, D, D, C.c_int]
+        L.orc_same_symmetry.argtypes = [D, D, D, D]
         _lib = L
     return _lib
 
@@ -226,3 +230,30 @@ def soc_block(mat, K, Kp, G, Gp) -> np.ndarray:
     lib().orc_soc_block(C.byref(cm), *args, _d(out))
     z = out[0::2] + 1j * out[1::2]
     return z.reshape(2, 2)
+
+
+def tetra_dos(pos, tet, E, G, band: int, n_points: int, e_min: float, e_max: float, reduce_bz_factor: float = 1.0,
+              spin_degeneracy: float = 2.0, in_wedge=None, wedge_only: bool = False, nthreads: int = 8):
+    """Tetrahedron DOS of one band (mesh_oracle.cpp): (energies, dos), each n_points long.
+
+    pos (nv, 3) in 1/m, tet (nt, 4) vertex indices, E (nv, nb) eV, G (nv, nb, 3) eV m."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    tet = np.ascontiguousarray(tet, dtype=np.int32)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    G = np.ascontiguousarray(G, dtype=np.float64)
+    wedge = None if in_wedge is None else np.ascontiguousarray(in_wedge, dtype=np.uint8)
+    en, dos = np.zeros(n_points), np.zeros(n_points)
+    st = lib().orc_tetra_dos(_d(pos), len(pos), _i(tet), len(tet),
+                             wedge.ctypes.data_as(C.POINTER(C.c_uint8)) if wedge is not None else None,
+                             _d(E), _d(G), E.shape[1], band, n_points, e_min, e_max, reduce_bz_factor,
+                             spin_degeneracy, int(wedge_only), _d(en), _d(dos), nthreads)
+    if st != 0:
+        raise ValueError("orc_tetra_dos: bad arguments")
+    return en, dos
+
+
+def same_symmetry(k_iw, k_bz, v) -> np.ndarray:
+    """apply_same_symmetry_operation restated (mesh_oracle.cpp)."""
+    out = np.zeros(3)
+    lib().orc_same_symmetry(*[_d(np.ascontiguousarray(a, dtype=np.float64)) for a in (k_iw, k_bz, v)], _d(out))
+    return out

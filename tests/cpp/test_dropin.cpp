@@ -6,6 +6,8 @@
 //       host-only checks: loader, k path, basis, exceptions (no device needed)
 //   test_dropin gpu <materials.yaml> <local-cohen.yaml> <outdir>
 //       band structures through the GPU; results written for the Python checker
+//   test_dropin mesh <local-cohen.yaml> <bz_mesh.msh> <outdir>
+//       apps/BandsOnBZ.cpp:85-140 then apps/DOS_MeshBZ.cpp:60-150 through MeshBZ
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -19,6 +21,10 @@
 #include "Hamiltonian.h"
 #include "Material.h"
 #include "SymmetryPoints.h"
+#include "bz_mesh.hpp"
+#include "csv_utils.hpp"
+
+#include <unistd.h>
 
 using namespace uepm::pseudopotential;
 
@@ -193,9 +199,88 @@ static int run_gpu(char** argv) {
     return g_fail ? 1 : 0;
 }
 
+static void write_matrix(const std::string& file, const std::vector<std::vector<double>>& rows) {
+    std::ofstream f(file);
+    f << std::setprecision(17);
+    for (const auto& r : rows) {
+        for (size_t i = 0; i < r.size(); ++i) f << r[i] << (i + 1 < r.size() ? " " : "\n");
+    }
+}
+
+static int run_mesh(char** argv) {
+    Materials cohen;
+    cohen.load_material_parameters(argv[2]);
+    const Material& si = cohen.materials.at("Si");
+    const std::string msh = argv[3];
+    if (chdir(argv[4]) != 0) return 2;  // the k-star file is looked up in the working directory
+    using uepm::mesh_bz::MeshBZ;
+    // the k-star file must exist, as in the reference; the mesh is kept to write it
+    {
+        MeshBZ m(si);
+        CHECK(throws<std::runtime_error>([&] { m.read_mesh_geometry_from_msh_file(msh); }));
+        m.write_kstar_ibz_to_bz("bz_mesh_kstar_ibz_to_bz.txt");
+    }
+    const int nrLevels = 4 + 12 + 4;
+    // BandsOnBZ, full zone
+    {
+        MeshBZ mesh(si);
+        mesh.set_number_threads_mesh_ops(8);
+        mesh.read_mesh_geometry_from_msh_file(msh);
+        CHECK(mesh.get_number_vertices() == 14705 && mesh.get_number_elements() == 73728);
+        CHECK(mesh.get_number_vertices_in_irreducible_wedge() == 405);
+        CHECK(std::fabs(mesh.get_reduce_bz_factor() - 1.0) < 1e-9);
+        BandStructure bs;
+        bs.Initialize(si, nrLevels, std::vector<Vector3D<double>>{}, 10, false, false);
+        mesh.compute_band_structure_over_mesh(bs, false);
+        CHECK(mesh.get_number_valence_bands() == 4 && mesh.get_number_conduction_bands() == 16);
+        write_matrix("mesh_E_full.txt", mesh.get_band_energies());
+        mesh.export_selected_bands_to_gmsh("bz_mesh_bands.msh", 4, 12, true, msh, true);
+        mesh.export_energies_and_gradients_to_vtk("bz_mesh_bands.vtk");
+        CHECK(throws<std::out_of_range>([&] { mesh.get_global_band_index(16, uepm::mesh_bz::MeshParticleType::conduction); }));
+    }
+    // BandsOnBZ -w: the wedge, fanned out
+    {
+        MeshBZ mesh(si);
+        mesh.read_mesh_geometry_from_msh_file(msh);
+        BandStructure bs;
+        bs.Initialize(si, nrLevels, std::vector<Vector3D<double>>{}, 10, false, false);
+        mesh.compute_band_structure_over_mesh(bs);  // use_iwedge defaults to true
+        CHECK(bs.get_band_energies().size() == 405);
+        write_matrix("mesh_E_iw.txt", mesh.get_band_energies());
+    }
+    // DOS_MeshBZ: read the bands back, shift, scissor, DOS per band, CSV
+    {
+        MeshBZ mesh(si);
+        mesh.read_mesh_geometry_from_msh_file(msh);
+        mesh.read_mesh_bands_from_msh_file("bz_mesh_bands.msh", -1, -1, true, false);
+        mesh.apply_scissor(1.12);
+        const int nb = static_cast<int>(mesh.get_number_bands_total());
+        CHECK(nb == 16);
+        std::vector<std::vector<double>> cols;
+        std::vector<std::string> header;
+        for (int b = 0; b < nb; ++b) {
+            auto ed = mesh.compute_dos_band_at_band_auto(b, 200, false, false);
+            cols.push_back(ed[0]);
+            cols.push_back(ed[1]);
+            header.push_back("energy_band_" + std::to_string(b));
+            header.push_back("dos_band_" + std::to_string(b));
+        }
+        uepm::utils::export_multiple_vector_to_csv("DOS_bz_mesh_bands.csv", header, cols);
+        const double d = mesh.compute_dos_at_energy_and_band(cols[2][100], 1);
+        CHECK(d == cols[3][100]);
+        CHECK(mesh.compute_dos_at_energy_and_band(cols[2][100], 1, true) == 0.0);
+        CHECK(throws<std::out_of_range>([&] { mesh.compute_dos_band_at_band_auto(nb, 10); }));
+    }
+    std::printf("mesh checks: %s\n", g_fail ? "FAILED" : "ok");
+    return g_fail ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc >= 5 && std::string(argv[1]) == "mesh") return run_mesh(argv);
     if (argc >= 3 && std::string(argv[1]) == "cpu") return run_cpu(argc, argv);
     if (argc >= 5 && std::string(argv[1]) == "gpu") return run_gpu(argv);
-    std::fprintf(stderr, "usage: test_dropin cpu <materials.yaml> [...] | gpu <materials.yaml> <cohen.yaml> <outdir>\n");
+    std::fprintf(stderr,
+                 "usage: test_dropin cpu <materials.yaml> [...] | gpu <materials.yaml> <cohen.yaml> <outdir>"
+                 " | mesh <cohen.yaml> <bz_mesh.msh> <outdir>\n");
     return 2;
 }

@@ -77,3 +77,50 @@ def test_dropin_band_structures_vs_oracle(oracle, tmp_path):
     assert np.abs(raw - w).max() <= 1e-9
     assert np.abs(H @ V - V * raw).max() <= 1e-8
     assert np.abs(V.conj().T @ V - np.eye(59)).max() <= 1e-10
+
+
+@pytest.mark.gpu
+def test_dropin_mesh_bands_and_dos(gpu_ctx, tmp_path):
+    """BandsOnBZ and DOS_MeshBZ call sequences through the drop-in MeshBZ,
+    against the same steps through the C ABI."""
+    import gzip
+    import shutil
+
+    from ultimateepm_amd import api, mesh
+
+    msh = str(tmp_path / "bz_mesh.msh")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "bz_mesh.msh.gz"), "rb") as src, open(msh, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    print("case: drop-in MeshBZ, test_dropin mesh", flush=True)
+    r = subprocess.run([_bin(), "mesh", COHEN, msh, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr
+    si = load("Si", COHEN)
+    m = mesh.MeshBZ.read_mesh_geometry_from_msh_file(gpu_ctx, msh, si.get_lattice_constant_meter())
+    plan = api.Plan(gpu_ctx, si, api.basis_vectors(10), GRADIENTS, 20)
+    m.compute_band_structure_over_mesh(plan, use_iwedge=False)
+    E, G = m.bands()
+    E_full = np.loadtxt(tmp_path / "mesh_E_full.txt")
+    E_iw = np.loadtxt(tmp_path / "mesh_E_iw.txt")
+    assert np.array_equal(E_full, E)
+    assert np.abs(E_iw - E).max() <= 1e-9
+    views = mesh.MshFile(str(tmp_path / "bz_mesh_bands.msh")).views
+    assert [v[0] for v in views[:4]] == ["band_0", "grad_band_0", "band_1", "grad_band_1"] and len(views) == 32
+    assert np.array_equal(views[0][1], E[:, 3]) and np.array_equal(views[1][1], G[:, 3])
+    assert np.array_equal(views[8][1], E[:, 4])  # first conduction band after the 4 valence ones
+    vtk = open(tmp_path / "bz_mesh_bands.vtk").read()
+    assert vtk.count("SCALARS band_energy_") == 20 and vtk.count("VECTORS band_grad_") == 20
+    # DOS_MeshBZ: the same steps through the C ABI
+    d = mesh.MeshBZ.read_mesh_geometry_from_msh_file(gpu_ctx, msh, si.get_lattice_constant_meter())
+    d.read_mesh_bands_from_msh_file(str(tmp_path / "bz_mesh_bands.msh"), -1, -1, True, False)
+    d.apply_scissor(1.12)
+    csv = np.loadtxt(tmp_path / "DOS_bz_mesh_bands.csv", delimiter=",", skiprows=1)
+    header = open(tmp_path / "DOS_bz_mesh_bands.csv").readline().strip().split(",")
+    assert header[:2] == ["energy_band_0", "dos_band_0"] and header[-1] == "DumbColumn"
+    for b in range(16):
+        en, dos = d.compute_dos_band_at_band_auto(b, 200)
+        assert np.allclose(csv[:, 2 * b], en, rtol=1e-5, atol=1e-5)
+        assert np.allclose(csv[:, 2 * b + 1], dos, rtol=1e-5, atol=1e-6 * dos.max())
+    plan.close()
+    m.close()
+    d.close()

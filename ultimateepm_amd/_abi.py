@@ -136,6 +136,42 @@ def lib() -> C.CDLL:
     L.uepm_server_wait.argtypes = [P, U64, D, D, D]
     L.uepm_server_stats_get.argtypes = [P, C.POINTER(uepm_server_stats)]
     L.uepm_server_destroy.argtypes = [P]
+    CP, U8, PI64, PI32 = C.c_char_p, C.POINTER(C.c_uint8), C.POINTER(I64), C.POINTER(I32)
+    PU64 = C.POINTER(U64)
+    L.uepm_msh_read.argtypes = [CP, C.POINTER(P)]
+    L.uepm_msh_counts.argtypes = [P, PI64, PI64, PI32]
+    L.uepm_msh_nodes.argtypes = [P, PU64, D]
+    L.uepm_msh_tetrahedra.argtypes = [P, PI32]
+    L.uepm_msh_view_info.argtypes = [P, I32, C.c_char_p, I32, PI32]
+    L.uepm_msh_view_data.argtypes = [P, I32, D]
+    L.uepm_msh_free.argtypes = [P]
+    L.uepm_msh_free.restype = None
+    L.uepm_msh_append_node_view.argtypes = [CP, CP, CP, I32, PU64, I64, D]
+    L.uepm_bzmesh_create.argtypes = [P, D, I64, PI32, I64, U8, C.POINTER(P)]
+    L.uepm_bzmesh_read_msh.argtypes = [P, CP, C.c_double, C.c_int, C.POINTER(P)]
+    L.uepm_bzmesh_destroy.argtypes = [P]
+    L.uepm_bzmesh_counts.argtypes = [P, PI64, PI64, PI32]
+    L.uepm_bzmesh_geometry.argtypes = [P, D, PI32, U8, PU64, D, D]
+    L.uepm_bzmesh_load_kstar.argtypes = [P, CP]
+    L.uepm_bzmesh_write_kstar.argtypes = [P, CP]
+    L.uepm_bzmesh_wedge_vertices.argtypes = [P, PI64, PI64]
+    L.uepm_bzmesh_kpoints.argtypes = [P, C.c_int, D, PI64]
+    L.uepm_bzmesh_set_mesh_bands.argtypes = [P, C.c_int, D, D, I32]
+    L.uepm_bzmesh_compute_bands.argtypes = [P, P, C.c_int, D]
+    L.uepm_bzmesh_bands.argtypes = [P, D, D, PI32]
+    L.uepm_bzmesh_band_info.argtypes = [P, I32, PI32, PI64, D, D]
+    L.uepm_bzmesh_band_groups.argtypes = [P, PI64, PI64, PI64, PI64]
+    L.uepm_bzmesh_set_bands.argtypes = [P, D, D, I32]
+    L.uepm_bzmesh_set_bands_device.argtypes = [P, P, P, I32, I32, I32, P]
+    L.uepm_bzmesh_dos.argtypes = [P, I32, I32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, D, D]
+    L.uepm_bzmesh_dos_device.argtypes = [P, I32, I32, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, P,
+                                         P, P]
+    L.uepm_bzmesh_dos_auto.argtypes = [P, I32, I32, C.c_int, C.c_int, D, D]
+    L.uepm_bzmesh_export_msh.argtypes = [P, CP, I64, I64, C.c_int, CP, C.c_int]
+    L.uepm_bzmesh_export_vtk.argtypes = [P, CP]
+    L.uepm_bzmesh_read_bands_msh.argtypes = [P, CP, I32, I32, C.c_int, C.c_int]
+    L.uepm_bzmesh_apply_scissor.argtypes = [P, C.c_double]
+    L.uepm_adjust_values.argtypes = [D, I64, I32, C.c_int, C.c_int, D]
     _lib = L
     return L
 

@@ -68,4 +68,20 @@ hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dta
 // Device-to-device copy (16-byte aligned pointers take the vector path).
 hipError_t launch_copy_bytes(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);
 
+// Tetrahedron density of states on a Brillouin-zone mesh (tetra_dos.hip).
+// pos: nv x 3 (1/m); tet: nt x 4 vertex indices; det: nt (6 x signed volume).
+hipError_t launch_tetra_det(const double* pos, const int32_t* tet, int64_t nt, double* det, hipStream_t s);
+// Vertex-major energies E (nv x ld) / gradients G (nv x ld x 3, may be null),
+// bands [first, first + nb) -> band-major Eb (nb x nv) / Gb (nb x nv x 3).
+hipError_t launch_bands_to_band_major(const double* E, const double* G, int64_t nv, int ld, int first, int nb,
+                                      double* Eb, double* Gb, hipStream_t s);
+// Slices of the tetrahedron list per energy group (a function of nt only); the
+// partial-sum workspace holds tetra_dos_slices(nt, n_points) x n_points
+// doubles, the window nt int2.
+int tetra_dos_slices(int64_t nt, int n_points);
+hipError_t launch_tetra_dos(const double* pos, const int32_t* tet, const double* det, const uint8_t* in_wedge,
+                            int64_t nt, const double* Eb, const double* Gb, int n_points, double e_min, double e_max,
+                            double reduce_bz_factor, double spin_degeneracy, double wedge_factor, int2* window,
+                            double* partial, double* energies, double* dos, hipStream_t s);
+
 }  // namespace uepm

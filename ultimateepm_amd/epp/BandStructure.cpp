@@ -122,36 +122,19 @@ void BandStructure::Compute(bool compute_gradient) { solve(compute_gradient, tru
 
 void BandStructure::Compute_parallel(bool compute_gradient, int /*nb_threads*/) { solve(compute_gradient, false); }
 
-bool BandStructure::FindBandGap(const std::vector<std::vector<double>>& results, double& maxValValence,
-                                double& minValConduction) {
-    // BandStructure.cpp:228-262: first level pair (from level 2) with a gap above 0.35 eV
-    if (results.empty() || results.front().size() < 2) return false;
-    const unsigned nrLevels = static_cast<unsigned>(results.front().size());
-    maxValValence = std::numeric_limits<double>::lowest();
-    minValConduction = std::numeric_limits<double>::infinity();
-    double fallbackMaxVal = maxValValence;
-    for (unsigned lo = 2; lo + 1 < nrLevels; ++lo) {
-        maxValValence = std::numeric_limits<double>::lowest();
-        minValConduction = std::numeric_limits<double>::infinity();
-        for (const auto& p : results) {
-            maxValValence = std::max(maxValValence, p[lo]);
-            minValConduction = std::min(minValConduction, p[lo + 1]);
-        }
-        if (lo == 3) fallbackMaxVal = maxValValence;
-        if (maxValValence + 0.35 < minValConduction) return true;
-    }
-    maxValValence = fallbackMaxVal;
-    return false;
-}
-
 double BandStructure::AdjustValues(bool minConductionBandToZero) {
-    // BandStructure.cpp:203-226
-    double maxValValence = 0.0, minValConduction = 0.0, band_gap = 0.0;
-    if (FindBandGap(m_energies, maxValValence, minValConduction)) band_gap = minValConduction - maxValValence;
-    const size_t nval = m_enable_spin_orbit_coupling ? 8 : 4;
-    for (auto& row : m_energies)
-        for (size_t b = 0; b < row.size(); ++b)
-            row[b] -= (b >= nval && minConductionBandToZero) ? minValConduction : maxValValence;
+    // BandStructure.cpp:203-262, through uepm_adjust_values (every row has the same length)
+    if (m_energies.empty()) return 0.0;
+    const size_t nb = m_energies.front().size();
+    std::vector<double> flat;
+    flat.reserve(m_energies.size() * nb);
+    for (const auto& row : m_energies) flat.insert(flat.end(), row.begin(), row.end());
+    double band_gap = 0.0;
+    throw_on_error(uepm_adjust_values(flat.data(), static_cast<int64_t>(m_energies.size()), static_cast<int32_t>(nb),
+                                      m_enable_spin_orbit_coupling ? 1 : 0, minConductionBandToZero ? 1 : 0, &band_gap),
+                   "BandStructure::AdjustValues");
+    for (size_t k = 0; k < m_energies.size(); ++k)
+        std::copy(flat.begin() + (long)(k * nb), flat.begin() + (long)((k + 1) * nb), m_energies[k].begin());
     return band_gap;
 }
 

@@ -74,8 +74,6 @@ class BandStructure {
 
     double m_computation_time_s = -1.0;
 
-    static bool FindBandGap(const std::vector<std::vector<double>>& results, double& maxValValence,
-                            double& minValConduction);
     bool GenerateBasisVectors(unsigned int nearestNeighborsNumber);
     void solve(bool compute_gradient, bool clamp_rows);
 };
