@@ -350,6 +350,28 @@ int uepm_bzmesh_read_bands_msh(uepm_bzmesh* mesh, const char* path, int32_t n_co
 /* MeshBZ::apply_scissor (bz_mesh.cpp:444-461). */
 int uepm_bzmesh_apply_scissor(uepm_bzmesh* mesh, double value);
 
+/* ---- dielectric function ------------------------------------------------------
+ * DielectricFunction::compute_dielectric_function
+ * (src/EPP/DielectricFunction.cpp:83-165) over the k-points it is given (the
+ * reference's [offset, offset + nb_kpoints) slice): for each q (|q| <= 1e-15
+ * becomes (1e-15, 1e-15, 1e-15)) and energy E,
+ *   out[q][E] = sum_k sum_{c = 4..nb-1} sum_{v = 0..3} |x_c(k+q) . x_v(k)|^2
+ *               [ (D - E)/((D - E)^2 + eta^2) + (D + E)/((D + E)^2 + eta^2) ],
+ * D = E_c(k+q) - E_v(k), with the reference's bilinear overlap (Eigen's
+ * adjoint().dot() conjugates twice).  Eigenpairs from the plan, which must
+ * carry UEPM_VECTORS, no UEPM_SOC (SetMatrix(k, nonlocal) has SOC off), and
+ * nb > 4.  k, q, energies, out: host; out nq x n_energies. */
+int uepm_dielectric(uepm_plan* plan, const double* k, int64_t nk, const double* q, int64_t nq, const double* energies,
+                    int32_t n_energies, double eta_smearing, double* out);
+/* DielectricFunction::merge_results renormalisation (DielectricFunction.cpp:
+ * 167-214) in place: eps = 1 + (2 pi / |q|^2) S / total_nk (|q|^2 = 1e-14 at
+ * q = 0).  Host-only. */
+int uepm_dielectric_merge(const double* q, int64_t nq, int32_t n_energies, int64_t total_nk, double* eps);
+/* DielectricFunction::apply_kramers_kronig (DielectricFunction.cpp:216-250):
+ * eps_imag = K (eps_real - 1), K_ij = -(2/pi) i / (j^2 - i^2), K_ii = 0, per q
+ * row.  Host-only. */
+int uepm_kramers_kronig(const double* eps_real, int64_t nq, int32_t n_energies, double* eps_imag);
+
 /* BandStructure::AdjustValues (src/EPP/BandStructure.cpp:203-262) over
  * E (nk x nb) in place: the first level pair from level 2 up whose gap
  * exceeds 0.35 eV sets the valence maximum (fallback: level 3), which is

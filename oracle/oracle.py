@@ -68,6 +68,8 @@ def lib() -> C.CDLL:
         L.orc_tetra_dos.argtypes = [D, C.c_int64, I32, C.c_int64, U8, D, D, C.c_int, C.c_int, C.c_int, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_int, D, D, C.c_int]
         L.orc_same_symmetry.argtypes = [D, D, D, D]
+        L.orc_dielectric.argtypes = [P, I32, C.c_int, C.c_int, C.c_int, D, C.c_int64, D, C.c_int64, D, C.c_int,
+                                     C.c_double, D, C.c_int]
         _lib = L
     return _lib
 
@@ -256,4 +258,19 @@ def same_symmetry(k_iw, k_bz, v) -> np.ndarray:
     """apply_same_symmetry_operation restated (mesh_oracle.cpp)."""
     out = np.zeros(3)
     lib().orc_same_symmetry(*[_d(np.ascontiguousarray(a, dtype=np.float64)) for a in (k_iw, k_bz, v)], _d(out))
+    return out
+
+
+def dielectric(mat, G, non_local: bool, nb: int, k, q, energies, eta: float, nthreads: int = 8) -> np.ndarray:
+    """compute_dielectric_function restated (dielectric_oracle.cpp): S[q][E], nq x n_energies."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros((len(q), len(en)))
+    cm = mat.to_c()
+    st = lib().orc_dielectric(C.byref(cm), _i(G), len(G), int(non_local), nb, _d(k), len(k), _d(q), len(q), _d(en),
+                              len(en), eta, _d(out), nthreads)
+    if st != 0:
+        raise RuntimeError(f"orc_dielectric failed ({st})")
     return out

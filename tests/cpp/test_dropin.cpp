@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "BandStructure.h"
+#include "DielectricFunction.hpp"
 #include "Hamiltonian.h"
 #include "Material.h"
 #include "SymmetryPoints.h"
@@ -194,6 +195,39 @@ static int run_gpu(char** argv) {
                 worst = std::max(worst, std::fabs(a.get_band_energies()[i][j] - b.get_band_energies()[i][j]));
         std::printf("mixed vs fp64 max |dE| = %.3e eV\n", worst);
         CHECK(worst <= 1e-4);
+    }
+    // 6. DielectricFunction as apps/epsilon.cpp:195-367 drives it (one rank)
+    {
+        const Material& si = ms.materials.at("Si");
+        BandStructure bs;
+        bs.Initialize(si, 16, std::vector<Vector3D<double>>{}, 8, true, false);
+        DielectricFunction df(si, bs.get_basis_vectors(), 16);
+        df.generate_k_points_grid(9, 9, 9, 0.0, false);
+        const std::size_t nk = df.get_kpoints().size();
+        df.set_export_prefix(out + "/Si_");
+        df.set_qpoints({Vector3D<double>(0, 0, 0), Vector3D<double>(0.05, 0.02, 0.0)});
+        std::vector<double> en;
+        for (int i = 0; i < 101; ++i) en.push_back(0.2 * i);
+        df.set_energies(en);
+        df.set_offset_k_index(0);
+        df.set_nb_kpoints(nk);
+        df.set_non_local_epm(true);
+        df.compute_dielectric_function(0.05, 0);
+        CHECK(df.get_dielectric_function().size() == 2);
+        {
+            std::ofstream f(out + "/dielectric_k.txt");
+            f << std::setprecision(17);
+            for (const auto& k : df.get_kpoints()) f << k.X << " " << k.Y << " " << k.Z << "\n";
+        }
+        std::ofstream f(out + "/dielectric_raw.txt");
+        f << std::setprecision(17);
+        for (const auto& row : df.get_dielectric_function())
+            for (size_t e = 0; e < row.size(); ++e) f << row[e] << (e + 1 < row.size() ? " " : "\n");
+        auto merged = DielectricFunction::merge_results(df, {df.get_dielectric_function()}, {static_cast<int>(nk)});
+        merged.apply_kramers_kronig();
+        merged.export_dielectric_function("", true);
+        CHECK(std::ifstream(out + "/Si__00001_0.050000_0.020000_0.000000.csv").good());
+        CHECK(throws<std::runtime_error>([&] { DielectricFunction::merge_results(df, {}, {}); }));
     }
     std::printf("gpu checks: %s\n", g_fail ? "FAILED" : "ok");
     return g_fail ? 1 : 0;

@@ -48,6 +48,17 @@ def test_dropin_band_structures_vs_oracle(oracle, tmp_path):
     assert np.abs(ev - ev_ref).max() <= 1e-9
     files = [f for f in os.listdir(tmp_path) if f.startswith("EPM_Si_nb_bands_16_path_LGXWKULWXKG_size_basis_137")]
     assert len(files) == 1
+    # 6. DielectricFunction: raw sums against the oracle, the merged CSV
+    kd = np.loadtxt(tmp_path / "dielectric_k.txt")
+    raw = np.loadtxt(tmp_path / "dielectric_raw.txt")
+    qs = np.array([[0.0, 0.0, 0.0], [0.05, 0.02, 0.0]])
+    en = 0.2 * np.arange(101)
+    ref = oracle.dielectric(load("Si"), oracle.basis(8), True, 16, kd, qs, en, 0.05)
+    assert np.abs(raw[1] - ref[1]).max() <= 1e-8 * np.abs(ref[1]).max()
+    assert np.abs(raw[0]).max() < 1e-12 and np.abs(ref[0]).max() < 1e-12  # q -> 0, real-symmetric H
+    csv = np.loadtxt(tmp_path / "Si__00001_0.050000_0.020000_0.000000.csv", delimiter=",", skiprows=1)
+    eps = 1.0 + (2 * np.pi / (0.05 ** 2 + 0.02 ** 2)) * raw[1] / len(kd)
+    assert np.allclose(csv[:, 1], eps, rtol=1e-5)
     text = (tmp_path / files[0]).read_text().splitlines()
     assert text[0] == "# Material Si" and text[1] == "# NBands 16" and text[2] == "# Nonlocal No"
     assert text[5].startswith("band_0,band_1") and len(text) == 6 + 200

@@ -299,3 +299,29 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def dielectric(plan: Plan, k: np.ndarray, q: np.ndarray, energies: np.ndarray, eta: float) -> np.ndarray:
+    """DielectricFunction::compute_dielectric_function sums S[q][E] (uepm_dielectric)."""
+    k = np.ascontiguousarray(k, dtype=np.float64).reshape(-1, 3)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    out = np.zeros((len(q), len(en)))
+    check(lib().uepm_dielectric(plan.handle, _dp(k), len(k), _dp(q), len(q), _dp(en), len(en), eta, _dp(out)))
+    return out
+
+
+def dielectric_merge(q: np.ndarray, S: np.ndarray, total_nk: int) -> np.ndarray:
+    """merge_results renormalisation: 1 + (2 pi / |q|^2) S / total_nk (uepm_dielectric_merge)."""
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    eps = np.array(S, dtype=np.float64, order="C")
+    check(lib().uepm_dielectric_merge(_dp(q), len(q), eps.shape[1], total_nk, _dp(eps)))
+    return eps
+
+
+def kramers_kronig(eps_real: np.ndarray) -> np.ndarray:
+    """apply_kramers_kronig: K (eps_real - 1) per q row (uepm_kramers_kronig)."""
+    er = np.ascontiguousarray(eps_real, dtype=np.float64)
+    out = np.zeros_like(er)
+    check(lib().uepm_kramers_kronig(_dp(er), er.shape[0], er.shape[1], _dp(out)))
+    return out

@@ -19,7 +19,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 EPP_DIR = os.path.join(HERE, "epp")
 EPP_OUT = os.path.join(HERE, "libuepm_epp.so")
 EPP_SOURCES = ["Material.cpp", "yaml_reader.cpp", "device_context.cpp", "SymmetryPoints.cpp", "Hamiltonian.cpp",
-               "BandStructure.cpp", "bz_mesh.cpp"]
+               "BandStructure.cpp", "bz_mesh.cpp", "DielectricFunction.cpp"]
 DROPIN_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_dropin.cpp")
 DROPIN_TEST_BIN = os.path.join(BUILD_DIR, "test_dropin")
 CXX = os.environ.get("CXX", "g++")
@@ -27,8 +27,8 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip", "copy.hip", "tridiag_reg.hip", "backtransform.hip",
-                  "tetra_dos.hip"]
-HOST_SOURCES = ["uepm_abi.cpp", "runtime.cpp", "msh_io.cpp", "bzmesh.cpp"]
+                  "tetra_dos.hip", "dielectric.hip"]
+HOST_SOURCES = ["uepm_abi.cpp", "runtime.cpp", "msh_io.cpp", "bzmesh.cpp", "dielectric.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-munsafe-fp-atomics"]
