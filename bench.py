@@ -284,6 +284,60 @@ def mesh_leg(ctx, steps: int, n_energies: int = 1000, cpu: bool = True, cpu_samp
     return out
 
 
+def epsilon_grid(n: int) -> np.ndarray:
+    """DielectricFunction::generate_k_points_grid(n, n, n, 0, false)
+    (DielectricFunction.cpp:59-74): the grid points of [-1, 1]^3 in the first BZ."""
+    pts = []
+    for i in range(n - 1):
+        for j in range(n - 1):
+            for k in range(n - 1):
+                x, y, z = (-1.0 + 2.0 * i / (n - 1), -1.0 + 2.0 * j / (n - 1), -1.0 + 2.0 * k / (n - 1))
+                if abs(x) <= 1 and abs(y) <= 1 and abs(z) <= 1 and abs(x) + abs(y) + abs(z) <= 1.5:
+                    pts.append((x, y, z))
+    return np.array(pts)
+
+
+def dielectric_leg(ctx, nq: int = 2, cpu: bool = True, cpu_sample: int = 32) -> dict:
+    """The reference's epsilon example (examples/si.epsilon.yaml): Si local,
+    n = 10 (N = 137), 16 bands, 40^3 grid k-points in the first BZ, energies
+    0..20 eV in 1 meV steps (20,001), eta = 0.05 eV; per q-point: eigenpairs at
+    k + q, 48 overlaps per k, Lorentzian sums (k eigenpairs solved once)."""
+    from ultimateepm_amd import api
+    from ultimateepm_amd.material import load
+
+    mat = load("Si")
+    G = api.basis_vectors(10)
+    ks = epsilon_grid(40)
+    en = np.round(np.arange(0.0, 20.0 + 1e-9, 0.001), 12)
+    eta = 0.05
+    qs = np.array([[0.01 * (i + 1), 0.0, 0.0] for i in range(nq)])
+    plan = api.Plan(ctx, mat, G, api.VECTORS, 16)
+    api.dielectric(plan, ks[:256], qs[:1], en[:64], eta)  # warm-up: workspaces
+    t0 = time.perf_counter()
+    S = api.dielectric(plan, ks, qs, en, eta)
+    dt = time.perf_counter() - t0
+    plan.close()
+    nk = len(ks)
+    out = {"workload": f"Si local (materials.yaml), n=10 (N={len(G)}), 16 bands, {nk} grid k-points (40^3 in the "
+                       f"first BZ), {len(en)} energies, eta {eta}, {nq} q-points along x",
+           "kpoints": nk, "qpoints": nq, "energies": len(en), "seconds": round(dt, 3),
+           "kq_pairs_per_s": round(nk * nq / dt, 1),
+           "lorentz_terms_per_s": round(nk * nq * 48 * len(en) / dt, 1),
+           "finite": bool(np.all(np.isfinite(S)))}
+    if cpu:
+        from oracle import oracle
+
+        sel = ks[:: max(1, nk // cpu_sample)][:cpu_sample]
+        t0 = time.perf_counter()
+        oracle.dielectric(mat, G, False, 16, sel, qs[:1], en, eta, nthreads=16)
+        cpu_dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"kq_pairs_per_s": round(len(sel) / cpu_dt, 2), "cores": 16, "kind": "port",
+                               "sample": f"oracle at {len(sel)} grid k-points x 1 q, same energies (eigen-solves "
+                                         "at k and k+q included)"}
+        out["gpu_over_cpu"] = round(out["kq_pairs_per_s"] / out["cpu_baseline"]["kq_pairs_per_s"], 1)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,6 +357,8 @@ def main() -> int:
                     help="requests of the serving leg of the default run (0: skip)")
     ap.add_argument("--mesh-steps", type=int, default=3,
                     help="timed repetitions of the mesh leg of the default run (0: skip)")
+    ap.add_argument("--dielectric-q", type=int, default=2,
+                    help="q-points of the dielectric leg of the default run (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -458,6 +514,8 @@ def main() -> int:
             "hbm_budget_gib": sv["hbm_budget_gib"], "max_inflight": sv["max_inflight"]}
     if args.mesh_steps > 0:
         out["mesh"] = mesh_leg(ctx, args.mesh_steps, cpu=not args.no_cpu_baseline)
+    if args.dielectric_q > 0:
+        out["dielectric"] = dielectric_leg(ctx, args.dielectric_q, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)
