@@ -168,6 +168,7 @@ MIXED_TOL = 1e-4  # eV (0.1 meV): precision form vs the FP64 oracle
     ("Si", 12, False, 12),   # the headline shape: N = 181
     ("Ge", 10, True, 16),    # 2N = 274, Kramers pairs
     ("GaAs", 8, True, 12),   # 2N = 178, zincblende with SOC
+    ("Si", 10, False, 24),   # nb + 4 = 28 Ritz vectors: the two-tile MFMA refinement
 ])
 def test_mixed_precision_vs_oracle(uepm, oracle, gpu_ctx, sym, n, soc, nb):
     mat = _mat(sym)
@@ -192,3 +193,22 @@ def test_mixed_precision_vs_oracle(uepm, oracle, gpu_ctx, sym, n, soc, nb):
         # (zincblende GaAs lacks inversion, so its SOC bands are spin-split)
         pairs = ev.reshape(len(ks), -1, 2)
         assert np.abs(pairs[:, :, 0] - pairs[:, :, 1]).max() <= MIXED_TOL
+
+
+def test_mixed_precision_gradients_many_bands(uepm, oracle, gpu_ctx):
+    """Precision form with 20 bands (24 Ritz vectors) and gradients against the FP64 oracle."""
+    print("case: mixed Si n=8 nb=20 gradients", flush=True)
+    mat = _mat("Si")
+    G = uepm.basis_vectors(8)
+    ks = uepm.bz_sample(314, 40)
+    plan = uepm.Plan(gpu_ctx, mat, G, uepm.NONLOCAL | uepm.MIXED | uepm.GRADIENTS, 20)
+    ev, _, gr = plan.solve(ks)
+    w_o, _, g_o = oracle.solve(mat, G, uepm.NONLOCAL | uepm.GRADIENTS, 20, ks, nthreads=8)
+    assert np.abs(ev - w_o).max() <= MIXED_TOL
+    d = np.diff(w_o, axis=1)
+    ok = np.ones(w_o.shape, dtype=bool)
+    ok[:, :-1] &= d > 1e-3
+    ok[:, 1:] &= d > 1e-3
+    ok[:, -1] = False
+    # gradients carry the Ritz vectors' accuracy (complex64 subspace, FP64 Ritz step)
+    assert np.abs(gr - g_o)[ok].max() <= 1e-4 * np.abs(g_o).max()

@@ -15,6 +15,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <cstdlib>
 
 #include "epm_physics.hpp"
 #include "solver.hpp"
@@ -40,6 +41,15 @@ __device__ __forceinline__ void wsync() {
 }
 
 constexpr int kRefBS = 256;
+
+// UEPM_REFINE_V1=1 keeps nbr > 16 on the VALU refine_kernel (A/B reference).
+bool refine_mfma2_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("UEPM_REFINE_V1");
+        return !(v && v[0] == '1');
+    }();
+    return on;
+}
 constexpr int kNbrMax = 28;
 constexpr int kChunk = 64;
 
@@ -450,12 +460,140 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
     write_ritz<BS, 16>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
 }
 
+// 16 < nbr <= 32: the same scheme on two 16-band tiles.  A lane's MFMA column
+// is band l%16 of tile 0 and band 16 + l%16 of tile 1; the H tile (A operand)
+// is shared by both Y tiles, S and M take four 16 x 16 blocks each.  The
+// per-wave partials meet in one LDS slot, the waves adding in fixed order,
+// so the sums do not depend on scheduling.
+template <int BS>
+__global__ __launch_bounds__(BS, 2) void refine_mfma2_kernel(const double2* __restrict__ H, int n, int ld,
+                                                          const double2* __restrict__ Xall, int nbr, int nb,
+                                                          double* __restrict__ evals, double2* __restrict__ evecs,
+                                                          double* __restrict__ grads, DevParams P,
+                                                          const double* __restrict__ kpts,
+                                                          const int32_t* __restrict__ G) {
+    constexpr int NW = BS / 64;
+    // One partial slot filled by the waves in turn; once S and M are formed the
+    // slot is reused for the Jacobi work arrays Zm and Wt (66 KB of LDS in all,
+    // two workgroups per CU).
+    __shared__ __align__(16) double part[4][4][16][16];  // block (at, bt), S_re S_im M_re M_im
+    static_assert(sizeof(part) >= 2 * 32 * 32 * sizeof(double2), "Zm and Wt fit in the partial slot");
+    __shared__ double2 Mm[32 * 32];
+    __shared__ double2 Ss[32 * 32];
+    double2* Zm = reinterpret_cast<double2*>(&part[0][0][0][0]);
+    double2* Wt = Zm + 32 * 32;
+    __shared__ double th[32];
+    __shared__ int perm[32];
+    __shared__ double4 rot[17];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int64_t kk = blockIdx.x;
+    const double2* A = H + kk * (int64_t)ld * n;
+    const double2* X = Xall + kk * (int64_t)nbr * n;  // X[b * n + r]
+    bool bl[2];
+    const double2* Xb[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        bl[t] = 16 * t + l16 < nbr;
+        Xb[t] = X + (bl[t] ? 16 * t + l16 : 0) * (int64_t)n;
+    }
+
+    dbl4 acc[4][4];  // [block at*2+bt][S_re, S_im, M_re, M_im]
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[q][c] = dbl4{0, 0, 0, 0};
+    const int ntile = (n + 15) >> 4;
+    for (int t = wv; t < ntile; t += NW) {
+        const int row = 16 * t + l16;  // A-operand row of this lane
+        dbl4 yre[2], yim[2];
+#pragma unroll
+        for (int bt = 0; bt < 2; ++bt) yre[bt] = yim[bt] = dbl4{0, 0, 0, 0};
+        for (int c0 = 0; c0 < n; c0 += 4) {
+            const int col = c0 + l4;
+            const bool hc = row < n && col < n;
+            const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
+            const double hr = hc ? h.x : 0.0, hi = hc ? h.y : 0.0;
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt) {
+                const double2 x = Xb[bt][col < n ? col : 0];
+                const double xr = (bl[bt] && col < n) ? x.x : 0.0, xi = (bl[bt] && col < n) ? x.y : 0.0;
+                yre[bt] = mfma64(hr, xr, yre[bt]);
+                yre[bt] = mfma64(-hi, xi, yre[bt]);
+                yim[bt] = mfma64(hr, xi, yim[bt]);
+                yim[bt] = mfma64(hi, xr, yim[bt]);
+            }
+        }
+        // S, M over this tile's rows, chunk v = rows 16t + 4v .. 16t + 4v + 3
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * t + 4 * v + l4;
+            double xr[2], xi[2];
+#pragma unroll
+            for (int bt = 0; bt < 2; ++bt) {
+                const double2 x = Xb[bt][r < n ? r : 0];
+                xr[bt] = (bl[bt] && r < n) ? x.x : 0.0;
+                xi[bt] = (bl[bt] && r < n) ? x.y : 0.0;
+            }
+#pragma unroll
+            for (int at = 0; at < 2; ++at)
+#pragma unroll
+                for (int bt = 0; bt < 2; ++bt) {
+                    dbl4* a4 = acc[2 * at + bt];
+                    a4[0] = mfma64(xr[at], xr[bt], a4[0]);
+                    a4[0] = mfma64(xi[at], xi[bt], a4[0]);
+                    a4[1] = mfma64(xr[at], xi[bt], a4[1]);
+                    a4[1] = mfma64(-xi[at], xr[bt], a4[1]);
+                    a4[2] = mfma64(xr[at], yre[bt][v], a4[2]);
+                    a4[2] = mfma64(xi[at], yim[bt][v], a4[2]);
+                    a4[3] = mfma64(xr[at], yim[bt][v], a4[3]);
+                    a4[3] = mfma64(-xi[at], yre[bt][v], a4[3]);
+                }
+        }
+    }
+    // partials: wave rd writes (rd = 0) or adds its share in round rd
+    for (int rd = 0; rd < NW; ++rd) {
+        if (wv == rd) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        double& dst = part[q][c][4 * v + l4][l16];
+                        dst = rd == 0 ? acc[q][c][v] : dst + acc[q][c][v];
+                    }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < 4 * 256; e += BS) {
+        const int q = e >> 8, a = (e >> 4) & 15, b = e & 15;
+        const int at = q >> 1, bt = q & 1;
+        const int ga = 16 * at + a, gb = 16 * bt + b;
+        if (ga < nbr && gb < nbr) {
+            Ss[ga * 32 + gb] = make_double2(part[q][0][a][b], part[q][1][a][b]);
+            Mm[ga * 32 + gb] = make_double2(part[q][2][a][b], part[q][3][a][b]);
+        }
+    }
+    __syncthreads();  // part is free from here: Zm and Wt take its place
+    if (wv == 0) small_rayleigh_ritz<32>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+    __syncthreads();
+    write_ritz<BS, 32>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
+}
+
+const char* refine_kernel_name(int nbr) {
+    return nbr <= 16 ? "refine_mfma_kernel" : refine_mfma2_enabled() ? "refine_mfma2_kernel" : "refine_kernel";
+}
+
 hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr, int nb, double* devals,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                          int64_t nk, hipStream_t s, double2* dY) {
     if (nbr > kNbrMax || nb > nbr) return hipErrorInvalidValue;
     if (nbr <= 16)
         hipLaunchKernelGGL(refine_mfma_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
+                           nb, devals, devecs, dgrads, P, dk, dG);
+    else if (refine_mfma2_enabled())
+        hipLaunchKernelGGL(refine_mfma2_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
                            nb, devals, devecs, dgrads, P, dk, dG);
     else
         hipLaunchKernelGGL(refine_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, dY, nbr,

@@ -46,6 +46,9 @@ hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                          int64_t nk, hipStream_t s, double2* dY);
 
+// Name of the refinement kernel launch_refine runs for nbr Ritz vectors.
+const char* refine_kernel_name(int nbr);
+
 // Register-resident complex64 reduction for dim <= tridiag_reg_max_n(): same
 // outputs as launch_tridiag_blocked_c64 (reflectors in Vout column-major with ld).
 int tridiag_reg_max_n();

@@ -561,7 +561,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     p->stage_name[kTriEig] = "tri_eig_wave_kernel";
     p->stage_name[kBack] = p->bt_v1 ? "backtransform_kernel"
                                     : (mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>");
-    p->stage_name[kRefine] = nbr <= 16 ? "refine_mfma_kernel" : "refine_kernel";
+    p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
     for (int ci = 0; ci < nchunks; ++ci) {
         const int64_t c0 = (int64_t)ci * chunk;
         const int64_t c = std::min(chunk, nk - c0);
