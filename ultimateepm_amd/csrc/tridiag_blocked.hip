@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "solver.hpp"
+#include "wave_ops.hpp"
 
 namespace uepm {
 
@@ -73,10 +74,10 @@ __device__ __forceinline__ void cfms_c(cx<T>& acc, cx<T> a, cx<T> b) {
     acc.y = fma(-a.y, b.x, fma(a.x, b.y, acc.y));
 }
 
+// wave-wide sum, every lane gets it (DPP and row swaps: no LDS crossbar trips)
 template <typename T>
 __device__ __forceinline__ T wsum(T v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    return wave::sum(v);
 }
 
 // 1/x from the hardware reciprocal and two Newton steps (full double precision).
