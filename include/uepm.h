@@ -372,6 +372,44 @@ int uepm_dielectric_merge(const double* q, int64_t nq, int32_t n_energies, int64
  * row.  Host-only. */
 int uepm_kramers_kronig(const double* eps_real, int64_t nq, int32_t n_energies, double* eps_imag);
 
+/* ---- eigenstates on the mesh (BZ_States) ------------------------------------
+ * BZ_States::compute_eigenstates and compute_shifted_eigenstates
+ * (src/BZ_MESH/bz_states.cpp:24-85): eigenpairs at every mesh vertex,
+ * k = position / (2 pi / a) with a the plan's lattice constant, or at
+ * k + q_shift (q_shift in reduced units; NULL selects the unshifted set).
+ * The plan must carry UEPM_VECTORS and no UEPM_SOC (the reference calls
+ * SetMatrix(k, false), local and SOC off, and keeps nb eigenvector columns).
+ * Eigenpairs stay on the plan's device inside the mesh object. */
+int uepm_bzmesh_compute_states(uepm_bzmesh* mesh, uepm_plan* plan, const double* q_shift);
+/* Host copies of a state set (shifted = 0: k, 1: k + q): evals nv x nb,
+ * evecs nv x nb x dim complex (interleaved re, im), each may be NULL; nb and
+ * dim may be NULL.  The reference keeps all dim eigenvalues per vertex; only
+ * the nb of the plan are computed here. */
+int uepm_bzmesh_states(const uepm_bzmesh* mesh, int shifted, double* evals, double* evecs, int32_t* nb, int32_t* dim);
+/* BZ_States::compute_dielectric_function (bz_states.cpp:95-164): per
+ * tetrahedron, |V|/4 times the sum over its four vertices of
+ *   sum_{c = 4..nb-1} sum_{v = 0..3} |x_c(k+q) . x_v(k)|^2
+ *     [ (D - E)/((D - E)^2 + eta^2) + (D + E)/((D + E)^2 + eta^2) ],
+ * D = E_c(k+q) - E_v(k), summed over the tetrahedra into S(E); then
+ * eps(E) = 1 + (q_e^2 / (eps_0 |q|^2)) / q_e * 2 / (2 pi)^3 * S(E) with q the
+ * shift in 1/m (|q| = 0 gives the reference's IEEE infinities).  Needs both
+ * state sets, nb > 4.  eps: n_energies; sums (S, may be NULL): n_energies;
+ * total_volume (sum of |V|, may be NULL). */
+int uepm_bzmesh_states_dielectric(uepm_bzmesh* mesh, const double* energies, int32_t n_energies, double eta_smearing,
+                                  double* eps, double* sums, double* total_volume);
+/* BZ_States::export_dielectric_function (bz_states.cpp:172-179):
+ * "<prefix>_dielectric_function.csv", header "Energy (eV),Dielectric function",
+ * rows "E,eps" in default stream formatting.  Host-only. */
+int uepm_write_dielectric_csv(const char* prefix, const double* energies, const double* eps, int32_t n_energies);
+/* BZ_States::export_full_eigenstates (bz_states.cpp:181-201): removes and
+ * recreates <root>/eigenstates (root NULL or "": the working directory) and
+ * writes per vertex k eigenvalues/eigenvalues_<k>.txt,
+ * eigenvalues/shiftedeigenvalues_<k>.txt (one row of nb values) and
+ * eigenvectors/eigenvectors_<k>.txt, eigenvectors/shiftedeigenvectors_<k>.txt
+ * (dim rows x nb columns of "(re,im)"), in Eigen's default stream format.
+ * The reference's eigenvalue rows hold all dim eigenvalues. */
+int uepm_bzmesh_export_states(const uepm_bzmesh* mesh, const char* root);
+
 /* BandStructure::AdjustValues (src/EPP/BandStructure.cpp:203-262) over
  * E (nk x nb) in place: the first level pair from level 2 up whose gap
  * exceeds 0.35 eV sets the valence maximum (fallback: level 3), which is

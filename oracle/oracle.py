@@ -70,6 +70,8 @@ def lib() -> C.CDLL:
         L.orc_same_symmetry.argtypes = [D, D, D, D]
         L.orc_dielectric.argtypes = [P, I32, C.c_int, C.c_int, C.c_int, D, C.c_int64, D, C.c_int64, D, C.c_int,
                                      C.c_double, D, C.c_int]
+        L.orc_bz_states_dielectric.argtypes = [P, I32, C.c_int, C.c_int, C.c_int, D, C.c_int64, I32, C.c_int64, D, D,
+                                               C.c_int, C.c_double, D, D, D, C.c_int]
         _lib = L
     return _lib
 
@@ -274,3 +276,21 @@ def dielectric(mat, G, non_local: bool, nb: int, k, q, energies, eta: float, nth
     if st != 0:
         raise RuntimeError(f"orc_dielectric failed ({st})")
     return out
+
+
+def bz_states_dielectric(mat, G, non_local: bool, nb: int, xyz, tet, q, energies, eta: float, nthreads: int = 8):
+    """BZ_States eigenstates + compute_dielectric_function restated (dielectric_oracle.cpp):
+    (eps, sums, total_volume) over the mesh (xyz in 1/m, tet nt x 4, q reduced)."""
+    G = np.ascontiguousarray(G, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    tet = np.ascontiguousarray(tet, dtype=np.int32).reshape(-1, 4)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(3)
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    eps, sums, vol = np.zeros(len(en)), np.zeros(len(en)), C.c_double()
+    cm = mat.to_c()
+    st = lib().orc_bz_states_dielectric(C.byref(cm), _i(G), len(G), int(non_local), nb, _d(xyz), len(xyz), _i(tet),
+                                        len(tet), _d(q), _d(en), len(en), eta, _d(eps), _d(sums), C.byref(vol),
+                                        nthreads)
+    if st != 0:
+        raise RuntimeError(f"orc_bz_states_dielectric failed ({st})")
+    return eps, sums, vol.value

@@ -8,6 +8,8 @@
 //       band structures through the GPU; results written for the Python checker
 //   test_dropin mesh <local-cohen.yaml> <bz_mesh.msh> <outdir>
 //       apps/BandsOnBZ.cpp:85-140 then apps/DOS_MeshBZ.cpp:60-150 through MeshBZ
+//   test_dropin states <cohen.yaml> <bz_mesh.msh> <outdir>
+//       apps/fullstates.cpp:42-104 through BZ_States
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -23,6 +25,7 @@
 #include "Material.h"
 #include "SymmetryPoints.h"
 #include "bz_mesh.hpp"
+#include "bz_states.hpp"
 #include "csv_utils.hpp"
 
 #include <unistd.h>
@@ -309,12 +312,74 @@ static int run_mesh(char** argv) {
     return g_fail ? 1 : 0;
 }
 
+// apps/fullstates.cpp:42-104: eigenstates at the vertices and at vertex + q,
+// the dielectric function over the tetrahedra, its CSV.
+static int run_states(char** argv) {
+    Materials materials;
+    materials.load_material_parameters(argv[2]);
+    const Material current_material = materials.materials.at("Si");
+    const std::string msh = argv[3];
+    if (chdir(argv[4]) != 0) return 2;
+    {
+        uepm::mesh_bz::MeshBZ m(current_material);
+        CHECK(throws<std::runtime_error>([&] { m.read_mesh_geometry_from_msh_file(msh); }));
+        m.write_kstar_ibz_to_bz("bz_mesh_kstar_ibz_to_bz.txt");
+    }
+    const int nb_bands_to_use = 12;
+    uepm::mesh_bz::BZ_States my_bz_mesh(current_material);
+    my_bz_mesh.set_nb_bands(nb_bands_to_use);
+    BandStructure band_structure{};
+    const int nb_nearest_neighbors = 10;
+    band_structure.Initialize(current_material, nb_bands_to_use, {}, nb_nearest_neighbors, false, false);
+    my_bz_mesh.set_basis_vectors(band_structure.get_basis_vectors());
+    my_bz_mesh.read_mesh_geometry_from_msh_file(msh);
+    std::cout << "Mesh volume: " << my_bz_mesh.compute_mesh_volume() << std::endl;
+    my_bz_mesh.compute_eigenstates(8);
+    Vector3D<double> q_shift = Vector3D<double>{3e-14, 0.0, 0.0};
+    my_bz_mesh.compute_shifted_eigenstates(q_shift, 8);
+    std::vector<double> list_energy;
+    const double min_energy = 0.0, max_energy = 20.0, energy_step = 0.01;
+    for (double energy = min_energy; energy <= max_energy + energy_step; energy += energy_step)
+        list_energy.push_back(energy);
+    my_bz_mesh.compute_dielectric_function(list_energy, 0.05, 8);
+    my_bz_mesh.export_dielectric_function("./TEST_DIELECTRIC_FUNCTION_");
+    CHECK(my_bz_mesh.get_dielectric_function_real().size() == list_energy.size());
+    const auto& X = my_bz_mesh.get_eigen_states();
+    CHECK(X.size() == 14705 && X[0].rows() == 137 && X[0].cols() == nb_bands_to_use);
+    const auto& E = my_bz_mesh.get_eigenvalues();
+    const auto& Eq = my_bz_mesh.get_eigenvalues(true);
+    CHECK(E.size() == 14705 && Eq.size() == 14705 && E[0].size() == nb_bands_to_use);
+    {
+        std::ofstream f("states_E.txt");
+        f << std::setprecision(17);
+        for (const auto& v : E) {
+            for (int j = 0; j < v.size(); ++j) f << (j ? " " : "") << v(j);
+            f << "\n";
+        }
+        double nrm = 0.0;
+        for (int i = 0; i < X[7].rows(); ++i) nrm += std::norm(X[7](i, 3));
+        CHECK(std::fabs(nrm - 1.0) < 1e-12);
+    }
+    CHECK(my_bz_mesh.compute_fermi_level(1e21, 300.0) == 0.0);
+    CHECK(my_bz_mesh.get_dielectric_function(Vector3D<double>{0.1, 0.0, 0.0}, 1.0) == 1.0);
+    {
+        uepm::mesh_bz::BZ_States no_basis(current_material);
+        no_basis.set_nb_bands(nb_bands_to_use);
+        no_basis.read_mesh_geometry_from_msh_file(msh);
+        CHECK(throws<std::invalid_argument>([&] { no_basis.compute_eigenstates(1); }));
+        CHECK(throws<std::invalid_argument>([&] { no_basis.compute_dielectric_function(list_energy, 0.05, 1); }));
+    }
+    std::printf("states checks: %s\n", g_fail ? "FAILED" : "ok");
+    return g_fail ? 1 : 0;
+}
+
 int main(int argc, char** argv) {
     if (argc >= 5 && std::string(argv[1]) == "mesh") return run_mesh(argv);
+    if (argc >= 5 && std::string(argv[1]) == "states") return run_states(argv);
     if (argc >= 3 && std::string(argv[1]) == "cpu") return run_cpu(argc, argv);
     if (argc >= 5 && std::string(argv[1]) == "gpu") return run_gpu(argv);
     std::fprintf(stderr,
                  "usage: test_dropin cpu <materials.yaml> [...] | gpu <materials.yaml> <cohen.yaml> <outdir>"
-                 " | mesh <cohen.yaml> <bz_mesh.msh> <outdir>\n");
+                 " | mesh <cohen.yaml> <bz_mesh.msh> <outdir> | states <cohen.yaml> <bz_mesh.msh> <outdir>\n");
     return 2;
 }

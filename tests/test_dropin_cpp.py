@@ -135,3 +135,42 @@ def test_dropin_mesh_bands_and_dos(gpu_ctx, tmp_path):
     plan.close()
     m.close()
     d.close()
+
+
+@pytest.mark.gpu
+def test_dropin_fullstates(gpu_ctx, tmp_path):
+    """apps/fullstates.cpp through the drop-in BZ_States, against the same
+    steps through the C ABI (same kernels, so the same bits)."""
+    import gzip
+    import shutil
+
+    from ultimateepm_amd import api, mesh
+
+    cohen = os.path.join(PARAMS_DIR, "materials-cohen.yaml")
+    msh = str(tmp_path / "bz_mesh.msh")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "bz_mesh.msh.gz"), "rb") as src, open(msh, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    print("case: drop-in BZ_States, test_dropin states", flush=True)
+    r = subprocess.run([_bin(), "states", cohen, msh, str(tmp_path)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:] + r.stderr
+    si = load("Si", cohen)
+    m = mesh.BZStates.read_mesh_geometry_from_msh_file(gpu_ctx, msh, si.get_lattice_constant_meter())
+    plan = api.Plan(gpu_ctx, si, api.basis_vectors(10), VECTORS, 12)
+    m.compute_eigenstates(plan)
+    m.compute_shifted_eigenstates(np.array([3e-14, 0.0, 0.0]), plan)
+    E, _ = m.eigen_states(vectors=False)
+    assert np.array_equal(np.loadtxt(tmp_path / "states_E.txt"), E)
+    en = []
+    e = 0.0
+    while e <= 20.0 + 0.01:
+        en.append(e)
+        e += 0.01
+    eps = m.compute_dielectric_function(np.array(en), 0.05)
+    csv = np.loadtxt(tmp_path / "TEST_DIELECTRIC_FUNCTION__dielectric_function.csv", delimiter=",", skiprows=1)
+    assert csv.shape == (len(en), 2)
+    assert np.allclose(csv[:, 0], en, rtol=1e-5, atol=1e-6)
+    assert np.allclose(csv[:, 1], eps, rtol=1e-5, atol=0.0)
+    assert "Total volume (k-space) integrated" in r.stdout and "EPS[0] = " in r.stdout
+    plan.close()
+    m.close()

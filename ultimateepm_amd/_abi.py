@@ -172,6 +172,11 @@ def lib() -> C.CDLL:
     L.uepm_bzmesh_read_bands_msh.argtypes = [P, CP, I32, I32, C.c_int, C.c_int]
     L.uepm_bzmesh_apply_scissor.argtypes = [P, C.c_double]
     L.uepm_adjust_values.argtypes = [D, I64, I32, C.c_int, C.c_int, D]
+    L.uepm_bzmesh_compute_states.argtypes = [P, P, D]
+    L.uepm_bzmesh_states.argtypes = [P, C.c_int, D, D, PI32, PI32]
+    L.uepm_bzmesh_states_dielectric.argtypes = [P, D, I32, C.c_double, D, D, D]
+    L.uepm_write_dielectric_csv.argtypes = [CP, D, D, I32]
+    L.uepm_bzmesh_export_states.argtypes = [P, CP]
     L.uepm_dielectric.argtypes = [P, D, I64, D, I64, D, I32, C.c_double, D]
     L.uepm_dielectric_merge.argtypes = [D, I64, I32, I64, D]
     L.uepm_kramers_kronig.argtypes = [D, I64, I32, D]

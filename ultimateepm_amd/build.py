@@ -19,7 +19,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 EPP_DIR = os.path.join(HERE, "epp")
 EPP_OUT = os.path.join(HERE, "libuepm_epp.so")
 EPP_SOURCES = ["Material.cpp", "yaml_reader.cpp", "device_context.cpp", "SymmetryPoints.cpp", "Hamiltonian.cpp",
-               "BandStructure.cpp", "bz_mesh.cpp", "DielectricFunction.cpp"]
+               "BandStructure.cpp", "bz_mesh.cpp", "DielectricFunction.cpp", "bz_states.cpp"]
 DROPIN_TEST_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "test_dropin.cpp")
 DROPIN_TEST_BIN = os.path.join(BUILD_DIR, "test_dropin")
 CXX = os.environ.get("CXX", "g++")

@@ -5,12 +5,17 @@
 // integrates the tetrahedron density of states (DOS_MeshBZ).  Host logic here
 // follows MeshBZ method by method (file:line at each function); vertex
 // positions, band energies and gradients also live on the device, where the
-// density of states runs (tetra_dos.hip).
+// density of states runs (tetra_dos.hip).  The eigenstates of BZ_States
+// (src/BZ_MESH/bz_states.cpp) at the vertices and at vertex + q stay on the
+// device too, for the tetrahedron-weighted dielectric function
+// (dielectric.hip).
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <limits>
 #include <map>
@@ -31,6 +36,14 @@ namespace {
 
 struct BandGroup {
     int64_t count = 0, start = 0;
+};
+
+// Eigenpairs at every vertex (BZ_States::m_eigenvalues_k / m_eigenvectors_k,
+// or the k + q pair), device-resident: E nv x nb, X nv x nb x dim complex.
+struct VertexStates {
+    bool set = false;
+    int nb = 0, dim = 0, device = 0;
+    DeviceBuffer E, X;
 };
 
 }  // namespace
@@ -64,6 +77,9 @@ struct uepm_bzmesh {
     int nb = 0;            // bands resident on the device
     bool dev_stale = true; // host bands changed since the last upload
     DeviceBuffer pos, dtet, det, wedge, Eb, Gb, stage, window, partial, out;
+    // BZ_States: [0] at the vertices, [1] at vertex + q; q in 1/m
+    VertexStates states[2];
+    double q_shift_si[3] = {0.0, 0.0, 0.0};
 };
 
 namespace {
@@ -918,6 +934,224 @@ int uepm_adjust_values(double* E, int64_t nk, int32_t nb, int soc, int min_cond_
                 e -= maxVal;
         }
     if (band_gap) *band_gap = gap;
+    return UEPM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- BZ_States
+
+namespace {
+constexpr int kStatesValence = 4;   // index_first_conduction_band (bz_states.cpp:97)
+constexpr int64_t kStatesChunk = 8192;  // vertices per dielectric pass
+constexpr double kQe = 1.602176634e-19;     // C (src/Common/physical_constants.hpp:20)
+constexpr double kEps0 = 8.8541878128e-12;  // F/m (physical_constants.hpp:31)
+
+// Eigen's default stream output of a dense matrix (IOFormat(): stream
+// precision, columns aligned to the widest coefficient, " " between
+// coefficients, "\n" between rows), coefficient (i, j) from at(i, j).
+template <class T, class At>
+void write_eigen_dense(std::ostream& os, int64_t rows, int64_t cols, At at) {
+    size_t width = 0;
+    for (int64_t j = 0; j < cols; ++j)
+        for (int64_t i = 0; i < rows; ++i) {
+            std::ostringstream one;
+            one.copyfmt(os);
+            one << static_cast<T>(at(i, j));
+            width = std::max(width, one.str().size());
+        }
+    for (int64_t i = 0; i < rows; ++i) {
+        for (int64_t j = 0; j < cols; ++j) {
+            if (j) os << ' ';
+            os.width(static_cast<std::streamsize>(width));
+            os << static_cast<T>(at(i, j));
+        }
+        if (i + 1 < rows) os << '\n';
+    }
+}
+}  // namespace
+
+extern "C" {
+
+// BZ_States::compute_eigenstates / compute_shifted_eigenstates
+// (bz_states.cpp:24-85): k = position / (2 pi / a) per vertex, plus q_shift
+// (reduced units) for the shifted set; eigenpairs from the plan's batched
+// solve, kept on the device.
+int uepm_bzmesh_compute_states(uepm_bzmesh* m, uepm_plan* p, const double* q_shift) {
+    if (!m || !p) return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_compute_states: null mesh or plan");
+    if (!(p->flags & UEPM_VECTORS) || (p->flags & UEPM_SOC))
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_compute_states: the plan needs UEPM_VECTORS and no UEPM_SOC");
+    const int which = q_shift ? 1 : 0;
+    VertexStates& st = m->states[which];
+    st.set = false;
+    HIP_OK(hipSetDevice(p->ctx->device));
+    const double normalization_factor = 2.0 * M_PI / p->P.a;
+    const int64_t nv = m->nv;
+    std::vector<double> k(3 * std::max<int64_t>(nv, 1));
+    for (int64_t i = 0; i < nv; ++i)
+        for (int c = 0; c < 3; ++c) {
+            double kc = (m->xyz[3 * i + c] * 1.0) / normalization_factor;
+            if (q_shift) kc += q_shift[c];
+            k[3 * i + c] = kc;
+        }
+    DeviceBuffer dk;
+    HIP_OK(dk.ensure(sizeof(double) * k.size()));
+    HIP_OK(st.E.ensure(sizeof(double) * (size_t)std::max<int64_t>(nv, 1) * p->nb));
+    HIP_OK(st.X.ensure(sizeof(double2) * (size_t)std::max<int64_t>(nv, 1) * p->nb * p->dim));
+    HIP_OK(hipMemcpy(dk.p, k.data(), sizeof(double) * 3 * nv, hipMemcpyHostToDevice));
+    const int rc = uepm_solve_device(p, dk.as<double>(), nv, st.E.as<double>(), st.X.as<double>(), nullptr, nullptr);
+    if (rc != UEPM_OK) return rc;
+    HIP_OK(hipDeviceSynchronize());
+    st.nb = p->nb;
+    st.dim = p->dim;
+    st.device = p->ctx->device;
+    st.set = true;
+    if (q_shift)
+        for (int c = 0; c < 3; ++c) m->q_shift_si[c] = q_shift[c] * normalization_factor;
+    return UEPM_OK;
+}
+
+int uepm_bzmesh_states(const uepm_bzmesh* m, int shifted, double* evals, double* evecs, int32_t* nb, int32_t* dim) {
+    if (!m) return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_states: null mesh");
+    const VertexStates& st = m->states[shifted ? 1 : 0];
+    if (!st.set) return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_states: eigenstates not computed");
+    if (nb) *nb = st.nb;
+    if (dim) *dim = st.dim;
+    if (evals || evecs) HIP_OK(hipSetDevice(st.device));
+    if (evals && m->nv)
+        HIP_OK(hipMemcpy(evals, st.E.p, sizeof(double) * (size_t)m->nv * st.nb, hipMemcpyDeviceToHost));
+    if (evecs && m->nv)
+        HIP_OK(hipMemcpy(evecs, st.X.p, sizeof(double2) * (size_t)m->nv * st.nb * st.dim, hipMemcpyDeviceToHost));
+    return UEPM_OK;
+}
+
+// BZ_States::compute_dielectric_function (bz_states.cpp:95-164): every
+// tetrahedron adds |V|/4 times the sums of its four vertices, so a vertex
+// enters with the weight w_v = sum of |V_t| / 4 over its tetrahedra (taken in
+// tetrahedron order); the weighted vertex sums run on the device.  Then
+// eps(E) = 1 + (q_e^2 / (eps_0 |q|^2) / q_e) (2 / (2 pi)^3) S(E), q in 1/m.
+int uepm_bzmesh_states_dielectric(uepm_bzmesh* m, const double* energies, int32_t n_energies, double eta_smearing,
+                                  double* eps, double* sums, double* total_volume) {
+    if (!m || n_energies < 0 || (n_energies > 0 && (!energies || !eps)))
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_states_dielectric: bad argument");
+    const VertexStates& a = m->states[0];
+    const VertexStates& b = m->states[1];
+    if (!a.set || !b.set)
+        return fail(UEPM_ERR_INVALID_ARG,
+                    "uepm_bzmesh_states_dielectric: compute the eigenstates at k and at k + q first");
+    if (a.nb != b.nb || a.dim != b.dim || a.device != b.device)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_states_dielectric: the two eigenstate sets differ in shape");
+    if (a.nb <= kStatesValence)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_states_dielectric: needs more than 4 bands (4 valence bands)");
+    const int64_t nv = m->nv;
+    std::vector<double> w(std::max<int64_t>(nv, 1), 0.0);
+    double tv = 0.0;
+    for (int64_t t = 0; t < m->nt; ++t) {
+        const int32_t* v = m->tet.data() + 4 * t;
+        const double vol = std::fabs(signed_volume(m->xyz.data(), v));
+        tv += vol;
+        for (int j = 0; j < 4; ++j) w[v[j]] += vol * 0.25;
+    }
+    if (total_volume) *total_volume = tv;
+    std::vector<double> S(std::max<int32_t>(n_energies, 1), 0.0);
+    if (n_energies > 0 && nv > 0) {
+        HIP_OK(hipSetDevice(a.device));
+        const int nb = a.nb, dim = a.dim;
+        const int64_t chunk = std::min<int64_t>(nv, kStatesChunk);
+        const int npair = (nb - kStatesValence) * kStatesValence;
+        DeviceBuffer dw, dwt, dd, part, sum, den;
+        HIP_OK(dw.ensure(sizeof(double) * nv));
+        HIP_OK(dwt.ensure(sizeof(double) * npair * chunk));
+        HIP_OK(dd.ensure(sizeof(double) * npair * chunk));
+        HIP_OK(part.ensure(sizeof(double) * (size_t)uepm::dielectric_slices(npair * chunk) * n_energies));
+        HIP_OK(sum.ensure(sizeof(double) * n_energies));
+        HIP_OK(den.ensure(sizeof(double) * n_energies));
+        HIP_OK(hipMemcpy(dw.p, w.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(den.p, energies, sizeof(double) * n_energies, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(sum.p, S.data(), sizeof(double) * n_energies, hipMemcpyHostToDevice));  // zeros
+        for (int64_t c0 = 0; c0 < nv; c0 += chunk) {
+            const int64_t n = std::min<int64_t>(chunk, nv - c0);
+            HIP_OK(uepm::launch_dielectric_accumulate(
+                a.X.as<double2>() + c0 * nb * (int64_t)dim, b.X.as<double2>() + c0 * nb * (int64_t)dim,
+                a.E.as<double>() + c0 * nb, b.E.as<double>() + c0 * nb, dw.as<double>() + c0, nb, dim, n,
+                den.as<double>(), n_energies, eta_smearing, dwt.as<double>(), dd.as<double>(), part.as<double>(),
+                sum.as<double>(), nullptr));
+        }
+        HIP_OK(hipDeviceSynchronize());
+        HIP_OK(hipMemcpy(S.data(), sum.p, sizeof(double) * n_energies, hipMemcpyDeviceToHost));
+    }
+    const double* q = m->q_shift_si;
+    const double len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+    const double q_squared = len * len;
+    const double coulomb_prefactor_eV = (kQe * kQe) / (kEps0 * q_squared) / kQe;
+    const double prefactor = coulomb_prefactor_eV * (2.0 / std::pow(2.0 * M_PI, 3));
+    for (int32_t e = 0; e < n_energies; ++e) {
+        if (sums) sums[e] = S[e];
+        eps[e] = 1.0 + prefactor * S[e];
+    }
+    return UEPM_OK;
+}
+
+// BZ_States::export_dielectric_function (bz_states.cpp:172-179):
+// "<prefix>_dielectric_function.csv", default stream formatting.
+int uepm_write_dielectric_csv(const char* prefix, const double* energies, const double* eps, int32_t n_energies) {
+    if (!prefix || n_energies < 0 || (n_energies > 0 && (!energies || !eps)))
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_write_dielectric_csv: bad argument");
+    std::ofstream f(std::string(prefix) + "_dielectric_function.csv");
+    if (!f) return fail(UEPM_ERR_RUNTIME, std::string("cannot open ") + prefix + "_dielectric_function.csv");
+    f << "Energy (eV),Dielectric function" << std::endl;
+    for (int32_t e = 0; e < n_energies; ++e) f << energies[e] << "," << eps[e] << std::endl;
+    return f ? UEPM_OK : fail(UEPM_ERR_RUNTIME, "uepm_write_dielectric_csv: write failed");
+}
+
+// BZ_States::export_full_eigenstates (bz_states.cpp:181-201): replaces
+// <root>/eigenstates with eigenvalues/{eigenvalues,shiftedeigenvalues}_<k>.txt
+// (one row) and eigenvectors/{eigenvectors,shiftedeigenvectors}_<k>.txt
+// (dim x nb, "(re,im)" coefficients), in Eigen's default stream format.
+int uepm_bzmesh_export_states(const uepm_bzmesh* m, const char* root) {
+    if (!m) return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_export_states: null mesh");
+    const VertexStates& a = m->states[0];
+    const VertexStates& b = m->states[1];
+    if (!a.set || !b.set)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_bzmesh_export_states: compute the eigenstates at k and at k + q first");
+    namespace fs = std::filesystem;
+    const fs::path base = fs::path(root && *root ? root : ".") / "eigenstates";
+    std::error_code ec;
+    fs::remove_all(base, ec);
+    fs::create_directory(base, ec);
+    fs::create_directory(base / "eigenvectors", ec);
+    fs::create_directory(base / "eigenvalues", ec);
+    if (ec) return fail(UEPM_ERR_RUNTIME, "uepm_bzmesh_export_states: cannot create " + base.string());
+    const int64_t nv = m->nv;
+    std::vector<double> E[2], X[2];
+    for (int s = 0; s < 2; ++s) {
+        const VertexStates& st = m->states[s];
+        E[s].resize((size_t)nv * st.nb);
+        X[s].resize((size_t)nv * st.nb * st.dim * 2);
+        HIP_OK(hipSetDevice(st.device));
+        if (nv) {
+            HIP_OK(hipMemcpy(E[s].data(), st.E.p, sizeof(double) * E[s].size(), hipMemcpyDeviceToHost));
+            HIP_OK(hipMemcpy(X[s].data(), st.X.p, sizeof(double) * X[s].size(), hipMemcpyDeviceToHost));
+        }
+    }
+    const char* vname[2] = {"eigenvalues_", "shiftedeigenvalues_"};
+    const char* xname[2] = {"eigenvectors_", "shiftedeigenvectors_"};
+    for (int64_t k = 0; k < nv; ++k) {
+        for (int s = 0; s < 2; ++s) {
+            const int nbs = m->states[s].nb, dims = m->states[s].dim;
+            std::ofstream fe(base / "eigenvalues" / (std::string(vname[s]) + std::to_string(k) + ".txt"));
+            const double* ev = E[s].data() + k * nbs;
+            write_eigen_dense<double>(fe, 1, nbs, [&](int64_t, int64_t j) { return ev[j]; });
+            fe << std::endl;
+            std::ofstream fx(base / "eigenvectors" / (std::string(xname[s]) + std::to_string(k) + ".txt"));
+            const double* xv = X[s].data() + k * nbs * (int64_t)dims * 2;
+            write_eigen_dense<std::complex<double>>(fx, dims, nbs, [&](int64_t i, int64_t j) {
+                return std::complex<double>(xv[2 * (j * dims + i)], xv[2 * (j * dims + i) + 1]);
+            });
+            fx << std::endl;
+            if (!fe || !fx) return fail(UEPM_ERR_RUNTIME, "uepm_bzmesh_export_states: write failed");
+        }
+    }
     return UEPM_OK;
 }
 

@@ -71,7 +71,7 @@ int uepm_dielectric(uepm_plan* p, const double* k, int64_t nk, const double* q, 
             st = uepm_solve_device(p, dkq.as<double>(), n, Ekq.as<double>(), Xkq.as<double>(), nullptr, s);
             if (st != UEPM_OK) return st;
             HIP_OK(uepm::launch_dielectric_accumulate(Xk.as<double2>(), Xkq.as<double2>(), Ek.as<double>(),
-                                                      Ekq.as<double>(), nb, dim, n, den.as<double>(), n_energies,
+                                                      Ekq.as<double>(), nullptr, nb, dim, n, den.as<double>(), n_energies,
                                                       eta_smearing, w.as<double>(), d.as<double>(),
                                                       part.as<double>(), sum.as<double>() + iq * n_energies, s));
         }

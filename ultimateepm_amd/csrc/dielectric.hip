@@ -13,10 +13,13 @@
 // Two kernels.  dielectric_overlap_kernel: one wave per k-point forms the
 // (nb - 4) x 4 overlaps from the band-contiguous eigenvectors of the two
 // solves (the valence vectors stay in L1/L2 across the conduction loop) and
-// writes weight and energy difference per pair.  dielectric_lorentz_kernel:
-// energy groups x slices of the pair list, per-thread partial sums in
-// registers, a fixed reduction order, partials per slice; the finish kernel
-// adds the slices in order into the running sum of the q row.
+// writes weight and energy difference per pair; an optional per-k factor
+// scales the weights (the tetrahedron volume share of a mesh vertex in
+// BZ_States::compute_dielectric_function, src/BZ_MESH/bz_states.cpp:95-164).
+// dielectric_lorentz_kernel: energy groups x slices of the pair list,
+// per-thread partial sums in registers, a fixed reduction order, partials per
+// slice; the finish kernel adds the slices in order into the running sum of
+// the q row.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -39,8 +42,8 @@ __device__ inline double wave_sum(double x) {
 
 __global__ void __launch_bounds__(kThreads) dielectric_overlap_kernel(
     const double2* __restrict__ Xk, const double2* __restrict__ Xkq, const double* __restrict__ Ek,
-    const double* __restrict__ Ekq, int nb, int dim, int64_t nk, double* __restrict__ weight,
-    double* __restrict__ delta) {
+    const double* __restrict__ Ekq, const double* __restrict__ kweight, int nb, int dim, int64_t nk,
+    double* __restrict__ weight, double* __restrict__ delta) {
     const int lane = threadIdx.x & 63;
     const int64_t k = blockIdx.x * (int64_t)(kThreads / 64) + (threadIdx.x >> 6);
     if (k >= nk) return;  // whole waves leave together; no block barrier follows
@@ -66,7 +69,7 @@ __global__ void __launch_bounds__(kThreads) dielectric_overlap_kernel(
             if (lane == 0) {
                 const double mag = hypot(sr, si);  // std::abs of the complex sum
                 const int64_t p = k * npair + (c - kValence) * kValence + v;
-                weight[p] = mag * mag;
+                weight[p] = kweight ? (mag * mag) * kweight[k] : mag * mag;
                 delta[p] = Ekq[k * nb + c] - Ek[k * nb + v];
             }
         }
@@ -131,13 +134,14 @@ int dielectric_slices(int64_t nitems) {
 }
 
 hipError_t launch_dielectric_accumulate(const double2* Xk, const double2* Xkq, const double* Ek, const double* Ekq,
-                                        int nb, int dim, int64_t nk, const double* energies, int n_energies,
+                                        const double* kweight, int nb, int dim, int64_t nk, const double* energies,
+                                        int n_energies,
                                         double eta, double* weight, double* delta, double* partial, double* sum,
                                         hipStream_t s) {
     if (nk <= 0) return hipSuccess;
     const unsigned ob = static_cast<unsigned>((nk + kThreads / 64 - 1) / (kThreads / 64));
-    hipLaunchKernelGGL(dielectric_overlap_kernel, dim3(ob), dim3(kThreads), 0, s, Xk, Xkq, Ek, Ekq, nb, dim, nk,
-                       weight, delta);
+    hipLaunchKernelGGL(dielectric_overlap_kernel, dim3(ob), dim3(kThreads), 0, s, Xk, Xkq, Ek, Ekq, kweight, nb, dim,
+                       nk, weight, delta);
     const int64_t nitems = nk * (nb - kValence) * kValence;
     const int slices = dielectric_slices(nitems);
     const unsigned groups = static_cast<unsigned>((n_energies + kEnergiesPerGroup - 1) / kEnergiesPerGroup);

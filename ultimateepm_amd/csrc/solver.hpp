@@ -89,11 +89,13 @@ hipError_t launch_tetra_dos(const double* pos, const int32_t* tet, const double*
 
 // Dielectric-function sums (dielectric.hip): eigenpairs at k (Ek, Xk) and
 // k + q (Ekq, Xkq), nk x nb and nk x nb x dim band-contiguous; adds the
-// n_energies Lorentzian sums into `sum`.  Workspace: weight and delta
+// n_energies Lorentzian sums into `sum`; kweight (nk doubles, or null for 1)
+// scales the pair weights of each k.  Workspace: weight and delta
 // nk x (nb - 4) x 4 doubles each, partial dielectric_slices(...) x n_energies.
 int dielectric_slices(int64_t nitems);
 hipError_t launch_dielectric_accumulate(const double2* Xk, const double2* Xkq, const double* Ek, const double* Ekq,
-                                        int nb, int dim, int64_t nk, const double* energies, int n_energies,
+                                        const double* kweight, int nb, int dim, int64_t nk, const double* energies,
+                                        int n_energies,
                                         double eta, double* weight, double* delta, double* partial, double* sum,
                                         hipStream_t s);
 

@@ -7,8 +7,9 @@
 // not needed: MSH 4.1 ASCII is read and written directly.
 //
 // Not carried over: octree search, random k draws, electron-phonon and
-// impact-ionisation rates, dielectric function on the mesh (consumers outside
-// the band solve), and the binary MSH format.
+// impact-ionisation rates (consumers outside the band solve), and the binary
+// MSH format.  The eigenstates and dielectric function of BZ_States derive
+// from this class (bz_states.hpp).
 #pragma once
 
 #include <cstddef>
@@ -84,7 +85,7 @@ class MeshBZ {
     std::vector<std::vector<Vector3D<double>>> get_energy_gradients() const;
     void print_band_info() const;
 
- private:
+ protected:
     uepm::pseudopotential::Material m_material;
     int m_nb_threads_mesh_ops = 1;
     uepm_bzmesh* m_mesh = nullptr;

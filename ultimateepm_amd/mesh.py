@@ -224,3 +224,59 @@ class MeshBZ:
 
     def apply_scissor(self, value: float):
         check(lib().uepm_bzmesh_apply_scissor(self._h, value))
+
+
+class BZStates(MeshBZ):
+    """uepm::mesh_bz::BZ_States (src/BZ_MESH/bz_states.hpp): eigenstates at the
+    mesh vertices and at vertex + q, kept on the device, and the tetrahedron-
+    weighted dielectric function (bz_states.cpp:95-164).  The plan passed to
+    the eigenstate calls plays the reference's Hamiltonian(material, basis)
+    with SetMatrix(k, false): UEPM_VECTORS, no SOC (local as in the reference
+    unless the plan says otherwise), nb = set_nb_bands."""
+
+    def __init__(self, ctx: api.Context | None, handle: C.c_void_p):
+        super().__init__(ctx, handle)
+        self.list_energies = np.zeros(0)
+        self.dielectric_function_real = np.zeros(0)
+        self.dielectric_sums = np.zeros(0)
+        self.total_volume_integrated = 0.0
+
+    def compute_eigenstates(self, plan: api.Plan):
+        check(lib().uepm_bzmesh_compute_states(self._h, plan.handle, None))
+
+    def compute_shifted_eigenstates(self, q_shift, plan: api.Plan):
+        q = np.ascontiguousarray(q_shift, dtype=np.float64).reshape(3)
+        check(lib().uepm_bzmesh_compute_states(self._h, plan.handle, _d(q)))
+
+    def eigen_states(self, shifted: bool = False, vectors: bool = True):
+        """(E nv x nb, X nv x nb x dim complex or None) of the k (or k + q) set."""
+        nb, dim = C.c_int32(), C.c_int32()
+        check(lib().uepm_bzmesh_states(self._h, int(shifted), None, None, C.byref(nb), C.byref(dim)))
+        nv = self.counts()[0]
+        E = np.zeros((nv, nb.value))
+        X = np.zeros((nv, nb.value, dim.value), dtype=np.complex128) if vectors else None
+        check(lib().uepm_bzmesh_states(self._h, int(shifted), _d(E),
+                                       X.ctypes.data_as(C.POINTER(C.c_double)) if vectors else None, None, None))
+        return E, X
+
+    def compute_dielectric_function(self, energies, eta_smearing: float) -> np.ndarray:
+        en = np.ascontiguousarray(energies, dtype=np.float64)
+        eps, sums, vol = np.zeros(len(en)), np.zeros(len(en)), C.c_double()
+        check(lib().uepm_bzmesh_states_dielectric(self._h, _d(en), len(en), eta_smearing, _d(eps), _d(sums),
+                                                  C.byref(vol)))
+        self.list_energies, self.dielectric_function_real, self.dielectric_sums = en, eps, sums
+        self.total_volume_integrated = vol.value
+        return eps
+
+    def export_dielectric_function(self, prefix: str):
+        check(lib().uepm_write_dielectric_csv(prefix.encode(), _d(self.list_energies),
+                                              _d(self.dielectric_function_real), len(self.list_energies)))
+
+    def export_full_eigenstates(self, root: str | None = None):
+        check(lib().uepm_bzmesh_export_states(self._h, root.encode() if root else None))
+
+    def compute_fermi_level(self, doping_concentration: float, temperature: float) -> float:
+        return 0.0  # a stub returning 0 in the reference (bz_states.cpp:166-169)
+
+    def get_dielectric_function(self, q, energy: float) -> complex:
+        return 1.0 + 0.0j  # a stub returning 1 in the reference (bz_states.hpp:87-90)
