@@ -338,6 +338,74 @@ def dielectric_leg(ctx, nq: int = 2, cpu: bool = True, cpu_sample: int = 32) -> 
     return out
 
 
+def states_leg(ctx, steps: int, cpu: bool = True, cpu_tetra: int = 200) -> dict:
+    """The reference's fullstates app (apps/fullstates.cpp): BZ_States over
+    bz_mesh.msh with Si (materials-cohen.yaml), n = 10 (N = 137), 12 bands,
+    eigenpairs at the 14,705 vertices and at vertex + q (q = (3e-14, 0, 0)),
+    then the tetrahedron-weighted dielectric function on 0..20 eV in 0.01 eV
+    steps, eta = 0.05 eV."""
+    import gzip
+    import shutil
+    import tempfile
+
+    from ultimateepm_amd import api, mesh
+    from ultimateepm_amd.material import PARAMS_DIR, load
+
+    mat = load("Si", os.path.join(PARAMS_DIR, "materials-cohen.yaml"))
+    tmp = tempfile.mkdtemp(prefix="uepm_states_")
+    path = os.path.join(tmp, "bz_mesh.msh")
+    with gzip.open(os.path.join(ROOT, "tests", "golden", "bz_mesh.msh.gz"), "rb") as src, open(path, "wb") as dst:
+        shutil.copyfileobj(src, dst)
+    m = mesh.BZStates.read_mesh_geometry_from_msh_file(ctx, path, mat.get_lattice_constant_meter())
+    shutil.rmtree(tmp, ignore_errors=True)
+    nv, nt, _ = m.counts()
+    G = api.basis_vectors(10)
+    q = np.array([3e-14, 0.0, 0.0])
+    en = []
+    e = 0.0
+    while e <= 20.0 + 0.01:  # fullstates.cpp:88-90
+        en.append(e)
+        e += 0.01
+    en = np.array(en)
+    out = {"workload": f"Si (materials-cohen.yaml), n=10 (N={len(G)}), 12 bands + eigenvectors at the {nv} vertices "
+                       f"of bz_mesh.msh and at vertex + q; dielectric function over its {nt} tetrahedra on "
+                       f"{len(en)} energies", "vertices": nv, "tetrahedra": nt, "energies": len(en)}
+    for mode in ("fp64", "mixed"):
+        plan = api.Plan(ctx, mat, G, api.VECTORS | (api.MIXED if mode == "mixed" else 0), 12)
+        m.compute_eigenstates(plan)  # warm-up: workspaces
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m.compute_eigenstates(plan)
+            m.compute_shifted_eigenstates(q, plan)
+        dt = (time.perf_counter() - t0) / steps
+        out[f"{mode}_states_kpts_per_s"] = round(2 * nv / dt, 1)
+        out[f"{mode}_states_ms"] = round(dt * 1e3, 3)
+        plan.close()
+    m.compute_dielectric_function(en, 0.05)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.compute_dielectric_function(en, 0.05)
+    dt = (time.perf_counter() - t0) / steps
+    out["dielectric_ms"] = round(dt * 1e3, 3)
+    out["lorentz_terms_per_s"] = round(nv * 32 * len(en) / dt, 1)
+    if cpu:
+        from oracle import oracle
+
+        geo = m.geometry()
+        tet = geo["tetrahedra"][:: max(1, nt // cpu_tetra)][:cpu_tetra]
+        ids, inv = np.unique(tet.ravel(), return_inverse=True)
+        t0 = time.perf_counter()
+        oracle.bz_states_dielectric(mat, G, False, 12, geo["xyz"][ids], inv.reshape(-1, 4), q, en, 0.05, nthreads=16)
+        cpu_dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"vertex_pairs_per_s": round(len(ids) / cpu_dt, 1), "cores": 16, "kind": "port",
+                               "sample": f"oracle over {len(tet)} tetrahedra ({len(ids)} vertices): eigenpairs at k "
+                                         "and k + q, then the tetrahedron sums, same energies"}
+        gpu_pairs = nv / (2 * nv / out["fp64_states_kpts_per_s"] + out["dielectric_ms"] / 1e3)
+        out["gpu_over_cpu"] = round(gpu_pairs / out["cpu_baseline"]["vertex_pairs_per_s"], 1)
+    m.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +427,8 @@ def main() -> int:
                     help="timed repetitions of the mesh leg of the default run (0: skip)")
     ap.add_argument("--dielectric-q", type=int, default=2,
                     help="q-points of the dielectric leg of the default run (0: skip)")
+    ap.add_argument("--states-steps", type=int, default=3,
+                    help="timed repeats of the BZ_States (fullstates) leg of the default run (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
@@ -516,6 +586,8 @@ def main() -> int:
         out["mesh"] = mesh_leg(ctx, args.mesh_steps, cpu=not args.no_cpu_baseline)
     if args.dielectric_q > 0:
         out["dielectric"] = dielectric_leg(ctx, args.dielectric_q, cpu=not args.no_cpu_baseline)
+    if args.states_steps > 0:
+        out["bz_states"] = states_leg(ctx, args.states_steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)
