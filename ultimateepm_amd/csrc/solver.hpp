@@ -52,8 +52,12 @@ const char* refine_kernel_name(int nbr);
 // Register-resident complex64 reduction for dim <= tridiag_reg_max_n(): same
 // outputs as launch_tridiag_blocked_c64 (reflectors in Vout column-major with ld).
 int tridiag_reg_max_n();
+// Trailing size the largest grid hands to the next smaller one (0: none) and
+// the complex64 hand-over workspace per k-point (bytes) `work` must provide.
+int tridiag_reg_split(int n);
+size_t tridiag_reg_work_bytes(int n);
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
-                                  float2* dtau, hipStream_t s);
+                                  float2* dtau, float2* work, hipStream_t s);
 
 // Eigenvectors of H from those of T: cluster re-orthogonalisation of Z
 // ([k][band][row], n x nb per k), then x = Q z with the reflectors of the

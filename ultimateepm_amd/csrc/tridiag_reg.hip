@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 
 #include "solver.hpp"
 
@@ -140,12 +141,20 @@ struct RegLayout {
 
 }  // namespace
 
-template <int PR, int PC, int LR, int LC, int MINWG>
-__global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double2* __restrict__ Hin, int n, int ld,
-                                                                    float2* __restrict__ Vout,
+// Stage arguments: steps s < s_end run here (s_end >= n - 1: the whole
+// reduction); when s_end < n - 1 the updated trailing block A[s_end:, s_end:]
+// (lower triangle, full diagonal) goes to Bout (complex64, leading dimension
+// ldb, ldb * ldb per k) for a smaller kernel to continue.  Input (leading
+// dimension ld), reflectors (ldv) and d, e, tau of k-point kk start at
+// kk * h_kstride, kk * v_kstride and kk * de_kstride.
+template <int PR, int PC, int LR, int LC, int MINWG, typename THin>
+__global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* __restrict__ Hin, int64_t h_kstride,
+                                                                    int n, int ld, float2* __restrict__ Vout,
+                                                                    int ldv, int64_t v_kstride,
                                                                     double* __restrict__ dout,
                                                                     double* __restrict__ eout,
-                                                                    float2* __restrict__ tauout) {
+                                                                    float2* __restrict__ tauout, int64_t de_kstride,
+                                                                    int s_end, float2* __restrict__ Bout, int ldb) {
     constexpr int BS = PR * PC;
     constexpr int NW = BS / 64;
     using Lay = RegLayout<PR, PC, LR, LC>;
@@ -166,11 +175,12 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int c = tid % PC, r = tid / PC;
     const int64_t kk = blockIdx.x;
-    const double2* H = Hin + kk * (int64_t)ld * n;
-    float2* Vk = Vout + kk * (int64_t)ld * n;
-    double* d = dout + kk * n;
-    double* e = eout + kk * n;
-    float2* tau_o = tauout + kk * n;
+    const THin* H = Hin + kk * h_kstride;
+    float2* Vk = Vout + kk * v_kstride;
+    double* d = dout + kk * de_kstride;
+    double* e = eout + kk * de_kstride;
+    float2* tau_o = tauout + kk * de_kstride;
+    const int s_stop = s_end < n - 1 ? s_end : n - 1;
 
     // ---- load the lower triangle (diagonal halved, strict upper zero) ----
     f2 A[NS];
@@ -181,7 +191,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
             const int i = r + PR * l, j = c + PC * m;
             f2 a = mk(0.f, 0.f);
             if (i < n && j < n && i >= j) {
-                const double2 h = H[i + (int64_t)j * ld];
+                const THin h = H[i + (int64_t)j * ld];
                 const float f = (i == j) ? 0.5f : 1.0f;
                 a = mk((float)h.x * f, (float)h.y * f);
             }
@@ -200,7 +210,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
         }
     const f2 one = mk(1.f, 0.f), zero = mk(0.f, 0.f);
 
-    for (int s = 0; s + 1 < n; ++s) {
+    for (int s = 0; s < s_stop; ++s) {
         const int cs = s % PC, ms = s / PC;
         const int l0 = (s + 1) / PR, m0 = (s + 1) / PC;  // first live row / column blocks
         // ---- a. column s to LDS (every row of the grid) ----
@@ -257,7 +267,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
             const int i = s + 2 + tid;
             if (i < n) {
                 const f2 v = mul(X[i], sc);
-                Vk[i + (int64_t)s * ld] = make_float2(v.x, v.y);
+                Vk[i + (int64_t)s * ldv] = make_float2(v.x, v.y);
             }
         }
         // v at this lane's rows and columns (X is zero outside s+1..n-1; loads
@@ -377,6 +387,22 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
             }
         }
     }
+    if (s_stop < n - 1) {
+        // hand the trailing block A[s_stop:, s_stop:] over (diagonal stored halved)
+        float2* B = Bout + kk * (int64_t)ldb * ldb;
+#pragma unroll
+        for (int l = 0; l < LR; ++l)
+#pragma unroll
+            for (int m = 0; m <= Lay::mmax(l); ++m) {
+                const int i = r + PR * l, j = c + PC * m;
+                if (i < n && j >= s_stop && i >= j) {
+                    const f2 a = A[Lay::idx(l, m)];
+                    const float f = (i == j) ? 2.0f : 1.0f;
+                    B[(i - s_stop) + (int64_t)(j - s_stop) * ldb] = make_float2(a.x * f, a.y * f);
+                }
+            }
+        return;
+    }
     // last diagonal element
     {
         const int s = n - 1;
@@ -397,16 +423,86 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const double
 // n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12 (rows x columns of the 32 x 16 grid)
 int tridiag_reg_max_n() { return 192; }
 
+// Hand-over between grids.  The late steps of a large grid carry its full
+// per-step overhead for a small trailing matrix; the 4 x 8 grid (128 VGPRs)
+// runs two workgroups per CU and the 2 x 4 grid three, so each grid stops
+// where the next smaller one can take the trailing block: 192-grid -> 128,
+// 128-grid -> 64.  UEPM_REG_SPLIT=0 keeps one grid, =1 only the first hand-over.
+namespace {
+int reg_split_mode() {
+    static const int mode = [] {
+        const char* v = std::getenv("UEPM_REG_SPLIT");
+        return v ? std::atoi(v) : 2;
+    }();
+    return mode;
+}
+constexpr int kTail1 = 128, kTail2 = 64;
+}  // namespace
+
+int tridiag_reg_split(int n) {
+    const int mode = reg_split_mode();
+    if (mode == 0) return 0;
+    if (n > kTail1) return kTail1;
+    if (mode >= 2 && n > kTail2) return kTail2;
+    return 0;
+}
+
+size_t tridiag_reg_work_bytes(int n) {
+    const int mode = reg_split_mode();
+    size_t b = 0;
+    if (mode != 0 && n > kTail1) b += sizeof(float2) * (size_t)kTail1 * kTail1;
+    if (mode >= 2 && n > kTail2) b += sizeof(float2) * (size_t)kTail2 * kTail2;
+    return b;
+}
+
+namespace {
+template <int LR, int LC, int MINWG, typename THin>
+void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, float2* V, int ldv, int64_t v_kstride,
+               double* dd, double* de, float2* dtau, int64_t de_kstride, int s_end, float2* B, int ldb, hipStream_t s) {
+    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, THin>), dim3((unsigned)nk), dim3(512), 0, s, H,
+                       h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
+}
+
+// The 128- and 64-grids on a block of size m <= their limit whose element
+// (0, 0) is the full matrix's (off, off): input H (ld, per-k stride hk),
+// outputs at offset `off` of the full matrix's V (ldv, stride vk) and d, e, tau
+// (stride dk).
+template <typename THin>
+void reg_tail(const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk, double* dd,
+              double* de, float2* dtau, int64_t dk, int off, float2* work2, hipStream_t s) {
+    float2* Vo = V + off + (int64_t)off * ldv;
+    if (m > kTail2) {
+        const bool hand = tridiag_reg_split(m) == kTail2;
+        reg_stage<4, 8, 4>(H, hk, m, ld, nk, Vo, ldv, vk, dd + off, de + off, dtau + off, dk,
+                           hand ? m - kTail2 : m, work2, kTail2, s);
+        if (!hand) return;
+        const int s1 = off + m - kTail2;
+        reg_stage<2, 4, 2>(work2, (int64_t)kTail2 * kTail2, kTail2, kTail2, nk, V + s1 + (int64_t)s1 * ldv, ldv, vk,
+                           dd + s1, de + s1, dtau + s1, dk, kTail2, nullptr, 0, s);
+    } else {
+        reg_stage<2, 4, 2>(H, hk, m, ld, nk, Vo, ldv, vk, dd + off, de + off, dtau + off, dk, m, nullptr, 0, s);
+    }
+}
+}  // namespace
+
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
-                                  float2* dtau, hipStream_t s) {
+                                  float2* dtau, float2* work, hipStream_t s) {
     if (n < 1 || n > 192 || nk <= 0) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)nk), block(512);
-    if (n <= 64)
-        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 2, 4, 2>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
-    else if (n <= 128)
-        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 4, 8, 2>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
-    else
-        hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, 6, 12, 1>), grid, block, 0, s, dH, n, ld, Vout, dd, de, dtau);
+    const int64_t hk = (int64_t)ld * n;
+    const int t = tridiag_reg_split(n);
+    if (t && !work) return hipErrorInvalidValue;
+    if (n <= kTail1) {
+        // the 128- or 64-grid on the whole matrix (64-grid hand-over inside reg_tail)
+        reg_tail(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, 0, work, s);
+    } else if (!t) {
+        reg_stage<6, 12, 1>(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n, nullptr, 0, s);
+    } else {
+        // steps 0 .. n-129 on the 6 x 12 grid, then the trailing 128 x 128 block
+        const int s0 = n - kTail1;
+        float2* work2 = work + nk * (int64_t)kTail1 * kTail1;
+        reg_stage<6, 12, 1>(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, s0, work, kTail1, s);
+        reg_tail(work, (int64_t)kTail1 * kTail1, kTail1, kTail1, nk, Vout, ld, hk, dd, de, dtau, n, s0, work2, s);
+    }
     return hipGetLastError();
 }
 

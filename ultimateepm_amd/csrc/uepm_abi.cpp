@@ -69,7 +69,8 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
     if (mixed) {
         z.Vf = sizeof(float2) * (size_t)n * n * chunk;
         z.tauf = sizeof(float2) * (size_t)n * chunk;
-        if (!p->tri_reg) z.S32 = sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk;
+        z.S32 = p->tri_reg ? uepm::tridiag_reg_work_bytes(n) * chunk
+                           : sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk;
         z.Ev = sizeof(double) * (size_t)nbr * chunk;
         z.X0 = sizeof(double2) * (size_t)n * nbr * chunk;
         if (nbr > 16) z.Y = sizeof(double2) * (size_t)n * nbr * chunk;
@@ -577,7 +578,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             HIP_OK(rec(ci, kTridiag, 0));
             if (p->tri_reg)
                 HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, p->d.as<double>(), p->e.as<double>(),
-                                                    p->tauf.as<float2>(), s));
+                                                    p->tauf.as<float2>(), p->S32.as<float2>(), s));
             else
                 HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
                                                         p->e.as<double>(), p->tauf.as<float2>(), s));
