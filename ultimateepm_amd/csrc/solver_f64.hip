@@ -5,9 +5,9 @@
 //                           src/EPP/Hamiltonian.cpp:50-127)
 //   (b) tridiag_kernel      Householder reduction H = Q T Q^H, T real symmetric
 //                           tridiagonal, one workgroup per k
-//   (c) tri_eig_kernel      bisection (Sturm counts) for the lowest nb eigenvalues
-//                           of T and inverse iteration for their vectors, one lane
-//                           per (k, band)
+//   (c) tri_eig_wave_kernel bisection (Sturm counts, multisection) for the lowest
+//                           nb eigenvalues of T and inverse iteration for their
+//                           vectors, one wave per k
 //   (d) backtransform_kernel  cluster re-orthogonalisation, x = Q z, unit norm,
 //                           Hellmann-Feynman gradients (Hamiltonian.cpp:146-194)
 //
@@ -242,20 +242,7 @@ __global__ __launch_bounds__(BS) void tridiag_kernel(double2* __restrict__ H, in
 }
 
 // ------------------------------------- (c) tridiagonal eigenvalues / vectors
-// One lane per (k, band).  Per-lane work arrays are interleaved across lanes
-// (element i of lane t at W[i * T + t]) so that lock-step lanes coalesce.
-__device__ __forceinline__ int sturm_count(const double* d, const double* e, int n, double x, double pivmin) {
-    double q = d[0] - x;
-    if (fabs(q) < pivmin) q = -pivmin;
-    int c = q < 0.0;
-    for (int i = 1; i < n; ++i) {
-        q = (d[i] - x) - (e[i - 1] * e[i - 1]) / q;
-        if (fabs(q) < pivmin) q = -pivmin;
-        c += q < 0.0;
-    }
-    return c;
-}
-
+// Start vector of inverse iteration: a fixed pseudo-random value per (row, band).
 __device__ __forceinline__ double lane_rand(uint32_t i, uint32_t b) {
     uint32_t h = i * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u;
     h ^= h >> 15;
@@ -264,117 +251,6 @@ __device__ __forceinline__ double lane_rand(uint32_t i, uint32_t b) {
     h *= 0x297A2D39u;
     h ^= h >> 15;
     return (double)(h >> 8) * (1.0 / 8388608.0) - 1.0;
-}
-
-__global__ void tri_eig_kernel(const double* __restrict__ dall, const double* __restrict__ eall, int n, int nb,
-                               int64_t nk, double* __restrict__ evals, double* __restrict__ Z, double* __restrict__ W,
-                               int want_vectors) {
-    const int64_t T = nk * nb;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= T) return;
-    const int64_t kk = t / nb;
-    const int b = (int)(t % nb);
-    const double* d = dall + kk * n;
-    const double* e = eall + kk * n;
-    // Gershgorin interval and scale
-    double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0, tnorm = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
-        const double er = i + 1 < n ? fabs(e[i]) : 0.0;
-        lo = fmin(lo, d[i] - el - er);
-        hi = fmax(hi, d[i] + el + er);
-        emax2 = fmax(emax2, er * er);
-        tnorm = fmax(tnorm, fabs(d[i]) + el + er);
-    }
-    const double pivmin = DBL_MIN * fmax(1.0, emax2);
-    const double eps = DBL_EPSILON;
-    lo -= 2.0 * eps * tnorm + pivmin;
-    hi += 2.0 * eps * tnorm + pivmin;
-    for (int it = 0; it < 200; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (hi - lo <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || mid <= lo || mid >= hi) break;
-        if (sturm_count(d, e, n, mid, pivmin) > b) hi = mid;
-        else lo = mid;
-    }
-    const double lam = 0.5 * (lo + hi);
-    evals[kk * nb + b] = lam;
-    if (!want_vectors) return;
-
-    // Inverse iteration with an LU factorisation of T - lam I with partial
-    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|).
-    double* L = W;                 // multipliers
-    double* U0 = W + (size_t)n * T;      // pivots
-    double* U1 = W + (size_t)2 * n * T;  // first super-diagonal
-    double* U2 = W + (size_t)3 * n * T;  // second super-diagonal
-    double* PV = W + (size_t)4 * n * T;  // interchange flags
-    double* X = W + (size_t)5 * n * T;   // iterate
-#define IDX(i) ((size_t)(i) * T + t)
-    const double tiny = eps * tnorm;
-    {
-        double a = d[0] - lam;
-        double c = n > 1 ? e[0] : 0.0;
-        for (int k = 0; k + 1 < n; ++k) {
-            const double sub = e[k];
-            const double dn = d[k + 1] - lam;
-            const double en = (k + 2 < n) ? e[k + 1] : 0.0;
-            if (fabs(a) >= fabs(sub)) {
-                if (a == 0.0) a = tiny;
-                const double mult = sub / a;
-                L[IDX(k)] = mult;
-                U0[IDX(k)] = a;
-                U1[IDX(k)] = c;
-                U2[IDX(k)] = 0.0;
-                PV[IDX(k)] = 0.0;
-                a = dn - mult * c;
-                c = en;
-            } else {
-                const double mult = a / sub;
-                L[IDX(k)] = mult;
-                U0[IDX(k)] = sub;
-                U1[IDX(k)] = dn;
-                U2[IDX(k)] = en;
-                PV[IDX(k)] = 1.0;
-                a = c - mult * dn;
-                c = -mult * en;
-            }
-        }
-        if (a == 0.0) a = tiny;
-        U0[IDX(n - 1)] = a;
-    }
-    for (int i = 0; i < n; ++i) X[IDX(i)] = lane_rand((uint32_t)i, (uint32_t)b);
-    for (int iter = 0; iter < 3; ++iter) {
-        // forward: apply interchanges and L
-        for (int k = 0; k + 1 < n; ++k) {
-            double yk = X[IDX(k)], yk1 = X[IDX(k + 1)];
-            if (PV[IDX(k)] != 0.0) {
-                const double tmp = yk;
-                yk = yk1;
-                yk1 = tmp;
-            }
-            yk1 -= L[IDX(k)] * yk;
-            X[IDX(k)] = yk;
-            X[IDX(k + 1)] = yk1;
-        }
-        // back substitution
-        double x2 = 0.0, x1 = X[IDX(n - 1)] / U0[IDX(n - 1)];
-        X[IDX(n - 1)] = x1;
-        double amax = fabs(x1);
-        for (int k = n - 2; k >= 0; --k) {
-            const double xk = (X[IDX(k)] - U1[IDX(k)] * x1 - U2[IDX(k)] * x2) / U0[IDX(k)];
-            X[IDX(k)] = xk;
-            x2 = x1;
-            x1 = xk;
-            amax = fmax(amax, fabs(xk));
-        }
-        const double s = 1.0 / amax;
-        for (int i = 0; i < n; ++i) X[IDX(i)] *= s;
-    }
-    double nrm = 0.0;
-    for (int i = 0; i < n; ++i) nrm += X[IDX(i)] * X[IDX(i)];
-    const double s = 1.0 / sqrt(nrm);
-    double* z = Z + (size_t)t * n;
-    for (int i = 0; i < n; ++i) z[i] = X[IDX(i)] * s;
-#undef IDX
 }
 
 // Wave-per-k variant: the d, e of a k-point are wave-uniform (scalar loads,
@@ -471,65 +347,68 @@ __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restri
     if (!want_vectors || !live || sub != 0) return;
 
     // Inverse iteration with an LU factorisation of T - lam I with partial
-    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|); the work arrays
-    // are interleaved over the nk * nb (k, band) pairs.
+    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|), three sweeps
+    // from a fixed pseudo-random start.  Each forward sweep recomputes the
+    // multipliers and interchanges (the same operations, so the same values)
+    // instead of reading them back, and each iterate's 1/max scaling is
+    // applied when the next sweep reads it: the interleaved work arrays (over
+    // the nk * nb (k, band) pairs) hold only the U factor and the iterate.
     const int64_t T = nk * nb;
     const int64_t t = kk * nb + b;
-    double* L = W;
-    double* U0 = W + (size_t)n * T;      // reciprocal pivots
-    double* U1 = W + (size_t)2 * n * T;  // first super-diagonal
-    double* U2 = W + (size_t)3 * n * T;  // second super-diagonal
-    double* PV = W + (size_t)4 * n * T;  // interchange flags
-    double* X = W + (size_t)5 * n * T;   // iterate
+    double* U0 = W;                      // reciprocal pivots
+    double* U1 = W + (size_t)n * T;      // first super-diagonal
+    double* U2 = W + (size_t)2 * n * T;  // second super-diagonal
+    double* X = W + (size_t)3 * n * T;   // forward result, then iterate (unscaled)
 #define IDX(i) ((size_t)(i) * T + t)
     const double tiny = eps * tnorm;
-    {
+    double sc = 1.0;  // scaling of the iterate in X
+    for (int iter = 0; iter < 3; ++iter) {
+        const bool first = iter == 0;
+        // forward: factor row k, apply the interchange and the multiplier
         double a = d[0] - lam;
         double c = n > 1 ? e[0] : 0.0;
+        double yk = first ? lane_rand(0u, (uint32_t)b) : X[IDX(0)] * sc;
         for (int k = 0; k + 1 < n; ++k) {
             const double sub_ = e[k];
             const double dn = d[k + 1] - lam;
             const double en = (k + 2 < n) ? e[k + 1] : 0.0;
+            double yk1 = first ? lane_rand((uint32_t)(k + 1), (uint32_t)b) : X[IDX(k + 1)] * sc;
+            double mult;
             if (fabs(a) >= fabs(sub_)) {
                 if (a == 0.0) a = tiny;
                 const double ra = rcp_nr64(a);
-                const double mult = sub_ * ra;
-                L[IDX(k)] = mult;
-                U0[IDX(k)] = ra;
-                U1[IDX(k)] = c;
-                U2[IDX(k)] = 0.0;
-                PV[IDX(k)] = 0.0;
+                mult = sub_ * ra;
+                if (first) {
+                    U0[IDX(k)] = ra;
+                    U1[IDX(k)] = c;
+                    U2[IDX(k)] = 0.0;
+                }
                 a = dn - mult * c;
                 c = en;
             } else {
                 const double rs = rcp_nr64(sub_);
-                const double mult = a * rs;
-                L[IDX(k)] = mult;
-                U0[IDX(k)] = rs;
-                U1[IDX(k)] = dn;
-                U2[IDX(k)] = en;
-                PV[IDX(k)] = 1.0;
+                mult = a * rs;
+                if (first) {
+                    U0[IDX(k)] = rs;
+                    U1[IDX(k)] = dn;
+                    U2[IDX(k)] = en;
+                }
                 a = c - mult * dn;
                 c = -mult * en;
-            }
-        }
-        if (a == 0.0) a = tiny;
-        U0[IDX(n - 1)] = rcp_nr64(a);
-    }
-    for (int i = 0; i < n; ++i) X[IDX(i)] = lane_rand((uint32_t)i, (uint32_t)b);
-    for (int iter = 0; iter < 3; ++iter) {
-        for (int k = 0; k + 1 < n; ++k) {
-            double yk = X[IDX(k)], yk1 = X[IDX(k + 1)];
-            if (PV[IDX(k)] != 0.0) {
                 const double tmp = yk;
                 yk = yk1;
                 yk1 = tmp;
             }
-            yk1 -= L[IDX(k)] * yk;
+            yk1 -= mult * yk;
             X[IDX(k)] = yk;
-            X[IDX(k + 1)] = yk1;
+            yk = yk1;
         }
-        double x2 = 0.0, x1 = X[IDX(n - 1)] * U0[IDX(n - 1)];
+        if (first) {
+            if (a == 0.0) a = tiny;
+            U0[IDX(n - 1)] = rcp_nr64(a);
+        }
+        // back substitution
+        double x2 = 0.0, x1 = yk * U0[IDX(n - 1)];
         X[IDX(n - 1)] = x1;
         double amax = fabs(x1);
         for (int k = n - 2; k >= 0; --k) {
@@ -539,14 +418,16 @@ __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restri
             x1 = xk;
             amax = fmax(amax, fabs(xk));
         }
-        const double sc = 1.0 / amax;
-        for (int i = 0; i < n; ++i) X[IDX(i)] *= sc;
+        sc = 1.0 / amax;
     }
     double nrm = 0.0;
-    for (int i = 0; i < n; ++i) nrm += X[IDX(i)] * X[IDX(i)];
+    for (int i = 0; i < n; ++i) {
+        const double xi = X[IDX(i)] * sc;
+        nrm += xi * xi;
+    }
     const double sn = 1.0 / sqrt(nrm);
     double* z = Z + (size_t)t * n;
-    for (int i = 0; i < n; ++i) z[i] = X[IDX(i)] * sn;
+    for (int i = 0; i < n; ++i) z[i] = (X[IDX(i)] * sc) * sn;
 #undef IDX
 }
 

@@ -79,7 +79,7 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
     }
     if (stage_vec) {
         z.Z = sizeof(double) * (size_t)n * nbr * chunk;
-        z.W = sizeof(double) * 6 * (size_t)n * nbr * chunk;
+        z.W = sizeof(double) * 4 * (size_t)n * nbr * chunk;  // U factor (3 rows) + iterate
         if (!mixed) z.X = sizeof(double2) * (size_t)n * nb * chunk;
     }
     return z;
