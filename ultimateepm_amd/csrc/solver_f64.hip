@@ -291,11 +291,14 @@ __device__ __forceinline__ int sturm_count_u(const double* __restrict__ d, const
     return c;
 }
 
+// abstol_rel: bisection also stops once a bracket is narrower than
+// abstol_rel * ||T||_inf (0: to the rounding level of the eigenvalue itself).
 template <int S>
 __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restrict__ dall,
                                                           const double* __restrict__ eall, int n, int nb, int64_t nk,
                                                           double* __restrict__ evals, double* __restrict__ Z,
-                                                          double* __restrict__ W, int want_vectors) {
+                                                          double* __restrict__ W, int want_vectors,
+                                                          double abstol_rel) {
     constexpr int NBW = 64 / S;  // bands per wave
     const int64_t kk = blockIdx.x;
     const int lane = threadIdx.x, sub = lane % S;
@@ -317,10 +320,11 @@ __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restri
     const double eps = DBL_EPSILON;
     lo -= 2.0 * eps * tnorm + pivmin;
     hi += 2.0 * eps * tnorm + pivmin;
+    const double abstol = abstol_rel * tnorm;
     // multisection: lane `sub` of a band probes lo + (sub+1) (hi-lo)/(S+1)
     for (int it = 0; it < 200; ++it) {
         const double w = hi - lo;
-        const bool done = !live || w <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin;
+        const bool done = !live || w <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || w <= abstol;
         if (__all(done)) break;
         const double x = lo + (double)(sub + 1) * (w / (double)(S + 1));
         const int cnt = done ? 0 : sturm_count_u(d, e, n, x, pivmin);
@@ -562,16 +566,16 @@ hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, do
 }
 
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
-                          double* dW, int want_vectors, hipStream_t s) {
+                          double* dW, int want_vectors, double abstol_rel, hipStream_t s) {
     if (nb <= 16)
         hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, want_vectors);
+                           dZ, dW, want_vectors, abstol_rel);
     else if (nb <= 32)
         hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, want_vectors);
+                           dZ, dW, want_vectors, abstol_rel);
     else
         hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), 0, s, dd,
-                           de, n, nb, nk, devals, dZ, dW, want_vectors);
+                           de, n, nb, nk, devals, dZ, dW, want_vectors, abstol_rel);
     return hipGetLastError();
 }
 

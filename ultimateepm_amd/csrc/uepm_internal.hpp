@@ -55,6 +55,8 @@ enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kRefine, kNumStages };
 // Extra Ritz vectors carried by the precision form beyond the nb requested,
 // so that a degenerate pair straddling band nb is refined as a whole.
 constexpr int kMixedExtra = 4;
+// Bisection bracket (relative to ||T||_inf) of the precision form's eigen-stage.
+constexpr double kMixedBisectTol = 1e-10;
 
 }  // namespace uepm_host
 
