@@ -56,9 +56,9 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
 
 // ------------------------------------------------------------ (a) assembly
 // One workgroup per k.  Per-basis-vector state in LDS, then every (i, j) pair
-// of the lower triangle: h_ij and h_ji are both evaluated as the reference
-// evaluates them and the stored value is 0.5 (h_ij + conj(h_ji)) (its
-// Diagonalize hermitisation), mirrored into the upper triangle.
+// of the lower triangle: the stored value is the reference's
+// 0.5 (h_ij + conj(h_ji)) (its Diagonalize hermitisation), mirrored into the
+// upper triangle; h_ji is derived from h_ij where that is exact (below).
 // Output: column-major dim x dim, leading dimension ld, one matrix per k.
 template <int BS>
 __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t* __restrict__ G,
@@ -73,18 +73,37 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
     __syncthreads();
     double2* A = H + (size_t)kk * ld * P.dim;
     const int npairs = N * (N + 1) / 2;
-    for (int t = threadIdx.x; t < npairs; t += BS) {
-        // t -> (i, j), i >= j, column-major over the packed lower triangle
-        int j = (int)((2.0 * N + 1.0 - sqrt((2.0 * N + 1.0) * (2.0 * N + 1.0) - 8.0 * t)) * 0.5);
+    // h_ji from h_ij: the phase arguments of V(G) and of the non-local term
+    // negate exactly under i <-> j, the device sincos is exactly odd / even
+    // there, and the square-well radial integrals are symmetric in |K_i|,
+    // |K_j| outside the near-equal branch of F_l (which reads one side's
+    // Bessel values) -- so one evaluation gives the reference's two-sided
+    // 0.5 (h_ij + conj(h_ji)) bit for bit.  Gaussian l = 2 wells and
+    // near-equal norms are evaluated both ways.
+    const bool swap_exact = !(P.well != 0 && (P.need_l2_a || P.need_l2_c));
+    // element t -> (i, j), column-major over the packed lower triangle; the
+    // first from the closed form, then advanced by BS per iteration
+    int t = threadIdx.x, j = 0, i = 0;
+    if (t < npairs) {
+        j = (int)((2.0 * N + 1.0 - sqrt((2.0 * N + 1.0) * (2.0 * N + 1.0) - 8.0 * t)) * 0.5);
         while (j > 0 && j * (2 * N - j + 1) / 2 > t) --j;
         while ((j + 1) * (2 * N - j) / 2 <= t) ++j;
-        const int i = j + (t - j * (2 * N - j + 1) / 2);
+        i = j + (t - j * (2 * N - j + 1) / 2);
+    }
+    for (; t < npairs; t += BS) {
+        if (t != (int)threadIdx.x) {
+            i += BS;
+            while (i >= N && j < N - 1) {  // past the end of column j: continue at the diagonal of column j + 1
+                i -= N - (j + 1);
+                ++j;
+            }
+        }
         const KState& si = ks[i];
         const KState& sj = ks[j];
         const int gx = G[3 * i] - G[3 * j], gy = G[3 * i + 1] - G[3 * j + 1], gz = G[3 * i + 2] - G[3 * j + 2];
-        double re_ij, im_ij, re_ji, im_ji;
+        double re_ij, im_ij;
         local_V(P, gx, gy, gz, re_ij, im_ij);
-        local_V(P, -gx, -gy, -gz, re_ji, im_ji);
+        double re_ji = re_ij, im_ji = -im_ij;
         if (i == j) {
             const double KG2 = P.kin * (si.kx * si.kx + si.ky * si.ky + si.kz * si.kz);
             re_ij += KG2;
@@ -93,7 +112,12 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
         if (P.nl) {
             double r1, i1, r2, i2;
             nonlocal(P, si, sj, r1, i1);
-            nonlocal(P, sj, si, r2, i2);
+            if (swap_exact && fabs(si.nK - sj.nK) > 1.0e-4) {
+                r2 = r1;
+                i2 = -i1;
+            } else {
+                nonlocal(P, sj, si, r2, i2);
+            }
             re_ij += r1; im_ij += i1;
             re_ji += r2; im_ji += i2;
         }
