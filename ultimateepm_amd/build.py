@@ -34,6 +34,11 @@ DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang
                 "-munsafe-fp-atomics"]
 
 
+# Per-file device flags.  refine.hip: MFMA accumulators in VGPRs (the default
+# heuristic shuttles them through AGPRs on every pass of the H X loop).
+FILE_FLAGS = {"refine.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
+
 def _run(cmd):
     print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
@@ -44,7 +49,8 @@ def build(verbose: bool = True) -> str:
     objs = []
     for src in DEVICE_SOURCES:
         obj = os.path.join(BUILD_DIR, src + ".o")
-        _run([HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, *FILE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+              "-o", obj])
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD_DIR, src + ".o")

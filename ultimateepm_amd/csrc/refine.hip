@@ -52,6 +52,9 @@ bool refine_mfma2_enabled() {
 }
 constexpr int kNbrMax = 28;
 constexpr int kChunk = 64;
+// column quads loaded ahead of their MFMAs in Y = H X (one / two band tiles)
+constexpr int kUnroll = 8;
+constexpr int kUnroll2 = 4;
 
 }  // namespace
 
@@ -402,17 +405,31 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
     for (int t = wv; t < ntile; t += NW) {
         const int row = 16 * t + l16;  // A-operand row of this lane
         dbl4 yre = {0, 0, 0, 0}, yim = yre;
-        for (int c0 = 0; c0 < n; c0 += 4) {
-            const int col = c0 + l4;
-            const bool hc = row < n && col < n;
-            const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
-            const double2 x = Xb[col < n ? col : 0];
-            const double hr = hc ? h.x : 0.0, hi = hc ? h.y : 0.0;
-            const double xr = (bl && col < n) ? x.x : 0.0, xi = (bl && col < n) ? x.y : 0.0;
-            yre = mfma64(hr, xr, yre);
-            yre = mfma64(-hi, xi, yre);
-            yim = mfma64(hr, xi, yim);
-            yim = mfma64(hi, xr, yim);
+        // kUnroll column quads per pass: all loads issue before the MFMAs that
+        // consume them (the accumulation order is still column-ascending)
+        for (int c0 = 0; c0 < n; c0 += 4 * kUnroll) {
+            double hr[kUnroll], hi[kUnroll], xr[kUnroll], xi[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                // out-of-range elements load an in-range address and are masked by a
+                // 0 / 1 factor (a select would let the compiler sink the load into a branch)
+                const int col = c0 + 4 * u + l4;
+                const bool hc = row < n && col < n;
+                const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
+                const double2 x = Xb[col < n ? col : 0];
+                const double mh = hc ? 1.0 : 0.0, mx = (bl && col < n) ? 1.0 : 0.0;
+                hr[u] = h.x * mh;
+                hi[u] = h.y * mh;
+                xr[u] = x.x * mx;
+                xi[u] = x.y * mx;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                yre = mfma64(hr[u], xr[u], yre);
+                yre = mfma64(-hi[u], xi[u], yre);
+                yim = mfma64(hr[u], xi[u], yim);
+                yim = mfma64(hi[u], xr[u], yim);
+            }
         }
         // S, M over this tile's rows, chunk v = rows 16t + 4v .. 16t + 4v + 3
 #pragma unroll
@@ -509,20 +526,33 @@ __global__ __launch_bounds__(BS, 2) void refine_mfma2_kernel(const double2* __re
         dbl4 yre[2], yim[2];
 #pragma unroll
         for (int bt = 0; bt < 2; ++bt) yre[bt] = yim[bt] = dbl4{0, 0, 0, 0};
-        for (int c0 = 0; c0 < n; c0 += 4) {
-            const int col = c0 + l4;
-            const bool hc = row < n && col < n;
-            const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
-            const double hr = hc ? h.x : 0.0, hi = hc ? h.y : 0.0;
+        for (int c0 = 0; c0 < n; c0 += 4 * kUnroll2) {
+            double hr[kUnroll2], hi[kUnroll2], xr[kUnroll2][2], xi[kUnroll2][2];
 #pragma unroll
-            for (int bt = 0; bt < 2; ++bt) {
-                const double2 x = Xb[bt][col < n ? col : 0];
-                const double xr = (bl[bt] && col < n) ? x.x : 0.0, xi = (bl[bt] && col < n) ? x.y : 0.0;
-                yre[bt] = mfma64(hr, xr, yre[bt]);
-                yre[bt] = mfma64(-hi, xi, yre[bt]);
-                yim[bt] = mfma64(hr, xi, yim[bt]);
-                yim[bt] = mfma64(hi, xr, yim[bt]);
+            for (int u = 0; u < kUnroll2; ++u) {
+                const int col = c0 + 4 * u + l4;
+                const bool hc = row < n && col < n;
+                const double2 h = A[(hc ? row : 0) + (int64_t)(hc ? col : 0) * ld];
+                const double mh = hc ? 1.0 : 0.0;
+                hr[u] = h.x * mh;
+                hi[u] = h.y * mh;
+#pragma unroll
+                for (int bt = 0; bt < 2; ++bt) {
+                    const double2 x = Xb[bt][col < n ? col : 0];
+                    const double mx = (bl[bt] && col < n) ? 1.0 : 0.0;
+                    xr[u][bt] = x.x * mx;
+                    xi[u][bt] = x.y * mx;
+                }
             }
+#pragma unroll
+            for (int u = 0; u < kUnroll2; ++u)
+#pragma unroll
+                for (int bt = 0; bt < 2; ++bt) {
+                    yre[bt] = mfma64(hr[u], xr[u][bt], yre[bt]);
+                    yre[bt] = mfma64(-hi[u], xi[u][bt], yre[bt]);
+                    yim[bt] = mfma64(hr[u], xi[u][bt], yim[bt]);
+                    yim[bt] = mfma64(hi[u], xr[u][bt], yim[bt]);
+                }
         }
         // S, M over this tile's rows, chunk v = rows 16t + 4v .. 16t + 4v + 3
 #pragma unroll
