@@ -424,35 +424,39 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
 int tridiag_reg_max_n() { return 192; }
 
 // Hand-over between grids.  The late steps of a large grid carry its full
-// per-step overhead for a small trailing matrix; the 4 x 8 grid (128 VGPRs)
-// runs two workgroups per CU and the 2 x 4 grid three, so each grid stops
-// where the next smaller one can take the trailing block: 192-grid -> 128,
-// 128-grid -> 64.  UEPM_REG_SPLIT=0 keeps one grid, =1 only the first hand-over.
+// per-step overhead for a small trailing matrix, so each grid stops where the
+// next smaller one can take the trailing block: 192-grid (6 x 12 blocks) ->
+// 160 (5 x 10) -> 128 (4 x 8, 128 VGPRs: two workgroups per CU) -> 64 (2 x 4,
+// three per CU).  UEPM_REG_SPLIT=0 keeps one grid.
 namespace {
 int reg_split_mode() {
     static const int mode = [] {
         const char* v = std::getenv("UEPM_REG_SPLIT");
-        return v ? std::atoi(v) : 2;
+        return v ? std::atoi(v) : 1;
     }();
     return mode;
 }
-constexpr int kTail1 = 128, kTail2 = 64;
+constexpr int kGrid[4] = {192, 160, 128, 64};
+// index of the smallest grid that holds n
+int grid_of(int n) {
+    int g = 0;
+    while (g < 3 && n <= kGrid[g + 1]) ++g;
+    return g;
+}
 }  // namespace
 
 int tridiag_reg_split(int n) {
-    const int mode = reg_split_mode();
-    if (mode == 0) return 0;
-    if (n > kTail1) return kTail1;
-    if (mode >= 2 && n > kTail2) return kTail2;
-    return 0;
+    if (reg_split_mode() == 0) return 0;
+    const int g = grid_of(n);
+    return g < 3 ? kGrid[g + 1] : 0;
 }
 
 size_t tridiag_reg_work_bytes(int n) {
-    const int mode = reg_split_mode();
-    size_t b = 0;
-    if (mode != 0 && n > kTail1) b += sizeof(float2) * (size_t)kTail1 * kTail1;
-    if (mode >= 2 && n > kTail2) b += sizeof(float2) * (size_t)kTail2 * kTail2;
-    return b;
+    // two ping-pong hand-over blocks: the first hand-over's size and the next one's
+    const int t1 = tridiag_reg_split(n);
+    if (!t1) return 0;
+    const int t2 = tridiag_reg_split(t1);
+    return sizeof(float2) * ((size_t)t1 * t1 + (size_t)t2 * t2);
 }
 
 namespace {
@@ -463,24 +467,14 @@ void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, floa
                        h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
 }
 
-// The 128- and 64-grids on a block of size m <= their limit whose element
-// (0, 0) is the full matrix's (off, off): input H (ld, per-k stride hk),
-// outputs at offset `off` of the full matrix's V (ldv, stride vk) and d, e, tau
-// (stride dk).
 template <typename THin>
-void reg_tail(const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk, double* dd,
-              double* de, float2* dtau, int64_t dk, int off, float2* work2, hipStream_t s) {
-    float2* Vo = V + off + (int64_t)off * ldv;
-    if (m > kTail2) {
-        const bool hand = tridiag_reg_split(m) == kTail2;
-        reg_stage<4, 8, 4>(H, hk, m, ld, nk, Vo, ldv, vk, dd + off, de + off, dtau + off, dk,
-                           hand ? m - kTail2 : m, work2, kTail2, s);
-        if (!hand) return;
-        const int s1 = off + m - kTail2;
-        reg_stage<2, 4, 2>(work2, (int64_t)kTail2 * kTail2, kTail2, kTail2, nk, V + s1 + (int64_t)s1 * ldv, ldv, vk,
-                           dd + s1, de + s1, dtau + s1, dk, kTail2, nullptr, 0, s);
-    } else {
-        reg_stage<2, 4, 2>(H, hk, m, ld, nk, Vo, ldv, vk, dd + off, de + off, dtau + off, dk, m, nullptr, 0, s);
+void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk,
+                double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
+    switch (g) {
+        case 0: reg_stage<6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<5, 10, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 2: reg_stage<4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: reg_stage<2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
 }  // namespace
@@ -489,19 +483,27 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
                                   float2* dtau, float2* work, hipStream_t s) {
     if (n < 1 || n > 192 || nk <= 0) return hipErrorInvalidValue;
     const int64_t hk = (int64_t)ld * n;
-    const int t = tridiag_reg_split(n);
-    if (t && !work) return hipErrorInvalidValue;
-    if (n <= kTail1) {
-        // the 128- or 64-grid on the whole matrix (64-grid hand-over inside reg_tail)
-        reg_tail(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, 0, work, s);
-    } else if (!t) {
-        reg_stage<6, 12, 1>(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n, nullptr, 0, s);
-    } else {
-        // steps 0 .. n-129 on the 6 x 12 grid, then the trailing 128 x 128 block
-        const int s0 = n - kTail1;
-        float2* work2 = work + nk * (int64_t)kTail1 * kTail1;
-        reg_stage<6, 12, 1>(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, s0, work, kTail1, s);
-        reg_tail(work, (int64_t)kTail1 * kTail1, kTail1, kTail1, nk, Vout, ld, hk, dd, de, dtau, n, s0, work2, s);
+    const int t1 = tridiag_reg_split(n);
+    if (!t1) {
+        grid_stage(grid_of(n), dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n, (float2*)nullptr, 0, s);
+        return hipGetLastError();
+    }
+    if (!work) return hipErrorInvalidValue;
+    // first grid on the full matrix, then each smaller grid on the trailing block
+    // of its size, in ping-pong blocks (the block of size t lives at
+    // buf[i] + k * t * t; a stage reads all of its block before it writes the next)
+    float2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
+    grid_stage(grid_of(n), dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n - t1, buf[0], t1, s);
+    int m = t1, cur = 0;
+    while (true) {
+        const int off = n - m;
+        const int t = tridiag_reg_split(m);
+        float2* Vo = Vout + off + (int64_t)off * ld;
+        grid_stage(grid_of(m), (const float2*)buf[cur], (int64_t)m * m, m, m, nk, Vo, ld, hk, dd + off, de + off,
+                   dtau + off, n, t ? m - t : m, t ? buf[cur ^ 1] : (float2*)nullptr, t, s);
+        if (!t) break;
+        m = t;
+        cur ^= 1;
     }
     return hipGetLastError();
 }
