@@ -426,8 +426,8 @@ int tridiag_reg_max_n() { return 192; }
 // Hand-over between grids.  The late steps of a large grid carry its full
 // per-step overhead for a small trailing matrix, so each grid stops where the
 // next smaller one can take the trailing block: 192-grid (6 x 12 blocks) ->
-// 160 (5 x 10) -> 128 (4 x 8, 128 VGPRs: two workgroups per CU) -> 64 (2 x 4,
-// three per CU).  UEPM_REG_SPLIT=0 keeps one grid.
+// 160 (5 x 10) -> 128 (4 x 8, 128 VGPRs: two workgroups per CU) -> 96 (3 x 6,
+// two per CU) -> 64 (2 x 4, three per CU).  UEPM_REG_SPLIT=0 keeps one grid.
 namespace {
 int reg_split_mode() {
     static const int mode = [] {
@@ -436,11 +436,12 @@ int reg_split_mode() {
     }();
     return mode;
 }
-constexpr int kGrid[4] = {192, 160, 128, 64};
+constexpr int kNumGrids = 5;
+constexpr int kGrid[kNumGrids] = {192, 160, 128, 96, 64};
 // index of the smallest grid that holds n
 int grid_of(int n) {
     int g = 0;
-    while (g < 3 && n <= kGrid[g + 1]) ++g;
+    while (g < kNumGrids - 1 && n <= kGrid[g + 1]) ++g;
     return g;
 }
 }  // namespace
@@ -448,7 +449,7 @@ int grid_of(int n) {
 int tridiag_reg_split(int n) {
     if (reg_split_mode() == 0) return 0;
     const int g = grid_of(n);
-    return g < 3 ? kGrid[g + 1] : 0;
+    return g < kNumGrids - 1 ? kGrid[g + 1] : 0;
 }
 
 size_t tridiag_reg_work_bytes(int n) {
@@ -474,6 +475,7 @@ void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, flo
         case 0: reg_stage<6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 1: reg_stage<5, 10, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 2: reg_stage<4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<3, 6, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         default: reg_stage<2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
