@@ -19,8 +19,13 @@ struct TriPlan {
     int64_t scratch_elems;  // complex elements of global scratch per workgroup
 };
 TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs);
+// s_end < n - 1 (a multiple of tridiag_blocked_panel()) stops after step
+// s_end - 1 and leaves the updated trailing block A[s_end:, s_end:] (lower
+// triangle) at its place in Vout, for launch_tridiag_reg_c128 to continue.
+int tridiag_blocked_panel();
 hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                      double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s);
+                                      double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s,
+                                      int s_end = 1 << 30);
 hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                       float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s);
 
@@ -61,6 +66,18 @@ int tridiag_reg_split(int n);
 size_t tridiag_reg_work_bytes(int n);
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s);
+// The same register-resident reduction in FP64 (grids 128, 96, 64): trailing
+// size it takes over for a matrix of size n (n itself when n <= 128; 0 with
+// UEPM_F64_REG=0), its hand-over workspace per k-point (bytes), and the
+// launch on a block of size m whose element (0, 0) sits at H (leading
+// dimension ld, per-k stride hk); reflectors, d, e, tau at the block's offset
+// of the full matrix's outputs (V may alias H: each stage loads its block
+// before it writes).
+int tridiag_reg_c128_tail(int n);
+size_t tridiag_reg_c128_work_bytes(int m);
+hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv,
+                                   int64_t vk, double* dd, double* de, double2* dtau, int64_t dk, double2* work,
+                                   hipStream_t s);
 
 // Eigenvectors of H from those of T: cluster re-orthogonalisation of Z
 // ([k][band][row], n x nb per k), then x = Q z with the reflectors of the

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                                                                  int c0, cx<T>* __restrict__ scratch,
                                                                  int64_t scratch_stride, cx<T>* __restrict__ Vout,
                                                                  double* __restrict__ dout, double* __restrict__ eout,
-                                                                 cx<T>* __restrict__ tauout) {
+                                                                 cx<T>* __restrict__ tauout, int s_end) {
     constexpr int MOD = 128 / (int)sizeof(cx<T>);
     constexpr int NW = BS / 64;
     constexpr int U = NBP / 2;  // K = 4 NBP real = 4 U per lane group
@@ -214,8 +214,11 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
     cx<T>* tau = tauout + kk * n;
     __syncthreads();
 
-    for (int p0 = 0; p0 < n - 1; p0 += NBP) {
-        const int nbp = min(NBP, n - 1 - p0);
+    // steps s < p_stop run here; s_end < n - 1 (a multiple of NBP) stops at a panel
+    // boundary, where the trailing matrix is fully updated, and hands it over
+    const int p_stop = min(n - 1, s_end);
+    for (int p0 = 0; p0 < p_stop; p0 += NBP) {
+        const int nbp = min(NBP, p_stop - p0);
         double tpr = 0.0, tpi = 0.0;  // tau of the previous step in this panel
         cx<T> nxt = zero;             // this thread's row of the next column (raw), prefetched
         bool have_nxt = false;
@@ -544,14 +547,19 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
         }
         __syncthreads();
     }
-    if (tid == 0) {
+    cx<T>* Vk = Vout + kk * (int64_t)ld * n;
+    if (p_stop < n - 1) {
+        // the updated trailing block A[p_stop:, p_stop:] (lower triangle) in place,
+        // at its own rows and columns of Vout, for the next reduction to continue
+        for (int c = p_stop; c < n; ++c)
+            for (int r = c + tid; r < n; r += BS) Vk[r + (int64_t)c * ld] = load(r, c);
+    } else if (tid == 0) {
         d[n - 1] = (double)load(n - 1, n - 1).x;
         e[n - 1] = 0.0;
         tau[n - 1] = zero;
     }
-    // reflectors out: column j rows j+2..n-1
-    cx<T>* Vk = Vout + kk * (int64_t)ld * n;
-    for (int c = 0; c + 2 < n; ++c)
+    // reflectors out: column j rows j+2..n-1, j < p_stop
+    for (int c = 0; c + 2 < n && c < p_stop; ++c)
         for (int r = c + 2 + tid; r < n; r += BS) Vk[r + (int64_t)c * ld] = load(r, c);
 }
 
@@ -580,33 +588,36 @@ TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs) {
     return tp;
 }
 
+int tridiag_blocked_panel() { return kTriNBP; }
+
 template <typename T, int BS>
 static hipError_t launch_tri_blocked_bs(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                        void* Vout, double* dd, double* de, void* dtau, hipStream_t s) {
+                                        void* Vout, double* dd, double* de, void* dtau, int s_end, hipStream_t s) {
     auto kern = tridiag_blocked_kernel<T, BS, kTriNBP>;
     hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(BS), tp.lds_bytes, s, dH, tp.n, ld, tp.c0,
                        static_cast<cx<T>*>(scratch), tp.scratch_elems, static_cast<cx<T>*>(Vout), dd, de,
-                       static_cast<cx<T>*>(dtau));
+                       static_cast<cx<T>*>(dtau), s_end);
     return hipGetLastError();
 }
 
 template <typename T>
 static hipError_t launch_tri_blocked_t(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                       void* Vout, double* dd, double* de, void* dtau, hipStream_t s) {
-    if (tp.bs >= 512) return launch_tri_blocked_bs<T, 512>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s);
-    return launch_tri_blocked_bs<T, 256>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s);
+                                       void* Vout, double* dd, double* de, void* dtau, int s_end, hipStream_t s) {
+    if (s_end < tp.n - 1 && (s_end < 0 || s_end % kTriNBP != 0)) return hipErrorInvalidValue;
+    if (tp.bs >= 512) return launch_tri_blocked_bs<T, 512>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
+    return launch_tri_blocked_bs<T, 256>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
 }
 
 hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                      double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s) {
-    return launch_tri_blocked_t<double>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s);
+                                      double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s, int s_end) {
+    return launch_tri_blocked_t<double>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
 }
 
 hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                       float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s) {
-    return launch_tri_blocked_t<float>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s);
+    return launch_tri_blocked_t<float>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, tp.n - 1, s);
 }
 
 }  // namespace uepm

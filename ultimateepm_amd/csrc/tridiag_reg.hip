@@ -31,65 +31,70 @@
 #include <cstdlib>
 
 #include "solver.hpp"
+#include "wave_ops.hpp"
 
 namespace uepm {
 
 namespace {
 
-struct f2 {
-    float x, y;
+// complex in the arithmetic type R (float for the precision form, double for FP64)
+template <typename R>
+struct c2 {
+    R x, y;
 };
-__device__ __forceinline__ f2 mk(float a, float b) { return {a, b}; }
-__device__ __forceinline__ f2 add(f2 a, f2 b) { return {a.x + b.x, a.y + b.y}; }
-__device__ __forceinline__ f2 mul(f2 a, f2 b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename R>
+__device__ __forceinline__ c2<R> mk(R a, R b) {
+    return {a, b};
+}
+template <typename R>
+__device__ __forceinline__ c2<R> add(c2<R> a, c2<R> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ c2<R> mul(c2<R> a, c2<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// c ? a : b per component (a select of the whole struct would go through memory)
+template <typename R>
+__device__ __forceinline__ c2<R> sel(bool c, c2<R> a, c2<R> b) {
+    return {c ? a.x : b.x, c ? a.y : b.y};
+}
 // acc += a * b
-__device__ __forceinline__ void fma_ab(f2& acc, f2 a, f2 b) {
-    acc.x = fmaf(a.x, b.x, fmaf(-a.y, b.y, acc.x));
-    acc.y = fmaf(a.x, b.y, fmaf(a.y, b.x, acc.y));
+template <typename R>
+__device__ __forceinline__ void fma_ab(c2<R>& acc, c2<R> a, c2<R> b) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
 }
 // acc += conj(a) * b
-__device__ __forceinline__ void fma_cab(f2& acc, f2 a, f2 b) {
-    acc.x = fmaf(a.x, b.x, fmaf(a.y, b.y, acc.x));
-    acc.y = fmaf(a.x, b.y, fmaf(-a.y, b.x, acc.y));
+template <typename R>
+__device__ __forceinline__ void fma_cab(c2<R>& acc, c2<R> a, c2<R> b) {
+    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
 }
 
-// Reductions: DPP inside a 16-lane row, VALU row swaps across the rows of a wave.
-template <int CTRL>
-__device__ __forceinline__ float dppv(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+// Reductions: DPP inside a 16-lane row, VALU row swaps across the rows of a
+// wave (wave_ops.hpp; 64-bit values move as two 32-bit halves).
+template <typename R>
+__device__ __forceinline__ void swap16(R& a, R& b) {
+    wave::Words<R> wa(a), wb(b);
+#pragma unroll
+    for (int i = 0; i < wave::Words<R>::n; ++i) wave::swap16(wa.w[i], wb.w[i]);
+    a = wa.value();
+    b = wb.value();
 }
-// sum over the 16 lanes of a row; every lane of the row gets the total
-__device__ __forceinline__ float row_sum16(float v) {
-    v += dppv<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dppv<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dppv<0x141>(v);  // row_half_mirror
-    v += dppv<0x140>(v);  // row_mirror
-    return v;
-}
-// gfx950 VALU row swaps, written out because the two-result builtins are not
-// usable here.  permlane16: a' = [a.R0 b.R0 a.R2 b.R2], b' = [a.R1 b.R1 a.R3 b.R3]
-// (R = 16-lane row); permlane32: a' = [a.lo b.lo], b' = [a.hi b.hi].  The s_nop
-// pairs keep VALU writes and reads of the operands apart from the swap.
-__device__ __forceinline__ void swap16(float& a, float& b) {
-    asm volatile("s_nop 2\n\tv_permlane16_swap_b32 %0, %1\n\ts_nop 2" : "+v"(a), "+v"(b));
-}
-__device__ __forceinline__ void swap32(float& a, float& b) {
-    asm volatile("s_nop 2\n\tv_permlane32_swap_b32 %0, %1\n\ts_nop 2" : "+v"(a), "+v"(b));
-}
-// sum over the wave, wave-uniform result
-__device__ __forceinline__ float wave_sum(float v) {
-    v = row_sum16(v);
-    float a = v, b = v;
-    swap16(a, b);
-    a += b;
-    float c = a, d = a;
-    swap32(c, d);
-    return c + d;
+template <typename R>
+__device__ __forceinline__ void swap32(R& a, R& b) {
+    wave::Words<R> wa(a), wb(b);
+#pragma unroll
+    for (int i = 0; i < wave::Words<R>::n; ++i) wave::swap32(wa.w[i], wb.w[i]);
+    a = wa.value();
+    b = wb.value();
 }
 // Reduce-scatter of two complex column partials over the four rows of a wave:
 // returns, per lane, row 0: Re p0, row 1: Im p0, row 2: Re p1, row 3: Im p1
 // (each summed over the four rows).
-__device__ __forceinline__ float cross_rows4_pair(float p0x, float p0y, float p1x, float p1y) {
+template <typename R>
+__device__ __forceinline__ R cross_rows4_pair(R p0x, R p0y, R p1x, R p1y) {
     swap16(p0x, p0y);  // rows 0,2: Re p0 parts; rows 1,3: Im p0 parts
     p0x += p0y;
     swap16(p1x, p1y);
@@ -98,18 +103,25 @@ __device__ __forceinline__ float cross_rows4_pair(float p0x, float p0y, float p1
     return p0x + p1x;
 }
 
-// 1/x: hardware reciprocal and one Newton step (full single precision)
-__device__ __forceinline__ float rcp1(float x) {
+// 1/x: hardware reciprocal and Newton steps to full precision
+__device__ __forceinline__ float rcp_full(float x) {
     const float r = __builtin_amdgcn_rcpf(x);
     return fmaf(fmaf(-x, r, 1.f), r, r);
 }
-
-__device__ __forceinline__ double rcp_nr(double x) {
+__device__ __forceinline__ double rcp_full(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(fma(-x, r, 1.0), r, r);
-    r = fma(fma(-x, r, 1.0), r, r);
-    return r;
+    return fma(fma(-x, r, 1.0), r, r);
 }
+
+template <typename VT, typename R>
+__device__ __forceinline__ VT store_c(R x, R y) {
+    VT v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+
 
 // Stored blocks of the cyclic lower triangle.
 template <int PR, int PC, int LR, int LC>
@@ -147,17 +159,18 @@ struct RegLayout {
 // ldb, ldb * ldb per k) for a smaller kernel to continue.  Input (leading
 // dimension ld), reflectors (ldv) and d, e, tau of k-point kk start at
 // kk * h_kstride, kk * v_kstride and kk * de_kstride.
-template <int PR, int PC, int LR, int LC, int MINWG, typename THin>
+template <int PR, int PC, int LR, int LC, int MINWG, typename R, typename VT, typename THin>
 __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* __restrict__ Hin, int64_t h_kstride,
-                                                                    int n, int ld, float2* __restrict__ Vout,
+                                                                    int n, int ld, VT* __restrict__ Vout,
                                                                     int ldv, int64_t v_kstride,
                                                                     double* __restrict__ dout,
                                                                     double* __restrict__ eout,
-                                                                    float2* __restrict__ tauout, int64_t de_kstride,
-                                                                    int s_end, float2* __restrict__ Bout, int ldb) {
+                                                                    VT* __restrict__ tauout, int64_t de_kstride,
+                                                                    int s_end, VT* __restrict__ Bout, int ldb) {
     constexpr int BS = PR * PC;
     constexpr int NW = BS / 64;
     using Lay = RegLayout<PR, PC, LR, LC>;
+    using f2 = c2<R>;
     constexpr int NS = Lay::count();
     constexpr int XC = Lay::XC;
     constexpr int NX = PR * LR;  // rows covered by the grid
@@ -170,16 +183,16 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     __shared__ f2 RP[NX];      // row sums of the symmetric product
     __shared__ f2 CP[NW][NX];  // column sums, one row per wave
     __shared__ f2 Y[NX];       // y = A v, zero at rows <= s and >= n
-    __shared__ float Q[NW];    // per-wave partial v^H A v
+    __shared__ R Q[NW];    // per-wave partial v^H A v
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int c = tid % PC, r = tid / PC;
     const int64_t kk = blockIdx.x;
     const THin* H = Hin + kk * h_kstride;
-    float2* Vk = Vout + kk * v_kstride;
+    VT* Vk = Vout + kk * v_kstride;
     double* d = dout + kk * de_kstride;
     double* e = eout + kk * de_kstride;
-    float2* tau_o = tauout + kk * de_kstride;
+    VT* tau_o = tauout + kk * de_kstride;
     const int s_stop = s_end < n - 1 ? s_end : n - 1;
 
     // ---- load the lower triangle (diagonal halved, strict upper zero) ----
@@ -189,26 +202,26 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
 #pragma unroll
         for (int m = 0; m <= Lay::mmax(l); ++m) {
             const int i = r + PR * l, j = c + PC * m;
-            f2 a = mk(0.f, 0.f);
+            f2 a = mk<R>(0, 0);
             if (i < n && j < n && i >= j) {
                 const THin h = H[i + (int64_t)j * ld];
-                const float f = (i == j) ? 0.5f : 1.0f;
-                a = mk((float)h.x * f, (float)h.y * f);
+                const R f = (i == j) ? R(0.5) : R(1);
+                a = mk<R>((R)h.x * f, (R)h.y * f);
             }
             A[Lay::idx(l, m)] = a;
         }
     }
     // update weights of the blocks that straddle the diagonal: 1 below, 1/2 on, 0 above
-    float HF[LR][XC];
+    R HF[LR][XC];
 #pragma unroll
     for (int l = 0; l < LR; ++l)
 #pragma unroll
         for (int q = 0; q < XC; ++q) {
             const int m = Lay::mcross(l) + q;
             const int dd = (r - c) + PR * l - PC * m;
-            HF[l][q] = dd > 0 ? 1.f : (dd == 0 ? 0.5f : 0.f);
+            HF[l][q] = dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0));
         }
-    const f2 one = mk(1.f, 0.f), zero = mk(0.f, 0.f);
+    const f2 one = mk<R>(1, 0), zero = mk<R>(0, 0);
 
     for (int s = 0; s < s_stop; ++s) {
         const int cs = s % PC, ms = s / PC;
@@ -224,7 +237,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                         const int i = r + PR * l;
                         if (m <= Lay::mmax(l)) {
                             const f2 a = A[Lay::idx(l, m)];
-                            X[i] = i > s ? a : zero;
+                            X[i] = sel(i > s, a, zero);
                             XD[i] = a;
                         } else {
                             X[i] = zero;  // block above the diagonal: rows < s
@@ -235,24 +248,24 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         }
         __syncthreads();
         // ---- b. reflector, redundantly in every wave ----
-        float part = 0.f;
+        R part = 0;
 #pragma unroll
         for (int q = 0; q < NXR; ++q) {
             const int i = s + 2 + lane + 64 * q;
             const f2 a = X[i < NX ? i : NX - 1];
-            const float t = fmaf(a.x, a.x, a.y * a.y);
-            part += i < NX ? t : 0.f;
+            const R t = fma(a.x, a.x, a.y * a.y);
+            part += i < NX ? t : R(0);
         }
-        const float xn2 = wave_sum(part);
+        const R xn2 = wave::sum(part);
         const f2 al = X[s + 1];
-        float beta = al.x, tr = 0.f, ti = 0.f, sr = 1.f, si = 0.f;
-        if (!(xn2 == 0.f && al.y == 0.f)) {
-            const float nrm = sqrtf(fmaf(al.x, al.x, fmaf(al.y, al.y, xn2)));
-            beta = al.x >= 0.f ? -nrm : nrm;
-            const float ib = rcp1(beta);
+        R beta = al.x, tr = 0, ti = 0, sr = 1, si = 0;
+        if (!(xn2 == R(0) && al.y == R(0))) {
+            const R nrm = sqrt(fma(al.x, al.x, fma(al.y, al.y, xn2)));
+            beta = al.x >= R(0) ? -nrm : nrm;
+            const R ib = rcp_full(beta);
             tr = (beta - al.x) * ib;
             ti = -al.y * ib;
-            const float br = al.x - beta, iden = rcp1(fmaf(br, br, al.y * al.y));
+            const R br = al.x - beta, iden = rcp_full(fma(br, br, al.y * al.y));
             sr = br * iden;
             si = -al.y * iden;
         }
@@ -261,13 +274,13 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         if (tid == 0) {
             d[s] = 2.0 * (double)XD[s].x;
             e[s] = (double)beta;
-            tau_o[s] = make_float2(tau.x, tau.y);
+            tau_o[s] = store_c<VT>(tau.x, tau.y);
         }
         {
             const int i = s + 2 + tid;
             if (i < n) {
                 const f2 v = mul(X[i], sc);
-                Vk[i + (int64_t)s * ldv] = make_float2(v.x, v.y);
+                Vk[i + (int64_t)s * ldv] = store_c<VT>(v.x, v.y);
             }
         }
         // v at this lane's rows and columns (X is zero outside s+1..n-1; loads
@@ -277,13 +290,13 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         for (int l = 0; l < LR; ++l) {
             const int i = r + PR * l;
             const f2 v = mul(X[i], sc);
-            vr[l] = i == s + 1 ? one : v;
+            vr[l] = sel(i == s + 1, one, v);
         }
 #pragma unroll
         for (int m = 0; m < LC; ++m) {
             const int j = c + PC * m;
             const f2 v = mul(X[j], sc);
-            vc[m] = j == s + 1 ? one : v;
+            vc[m] = sel(j == s + 1, one, v);
         }
         // ---- c. y = A v on the live blocks; local share of v^H A v ----
         f2 rp[LR], cp[LC];
@@ -303,14 +316,14 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
             }
         }
         // sum over stored elements of conj(v_i) A_ij v_j; v^H A v = 2 Re of its total
-        float qv = 0.f;
+        R qv = 0;
 #pragma unroll
         for (int l = 0; l < LR; ++l)
-            if (l >= l0) qv = fmaf(vr[l].x, rp[l].x, fmaf(vr[l].y, rp[l].y, qv));
+            if (l >= l0) qv = fma(vr[l].x, rp[l].x, fma(vr[l].y, rp[l].y, qv));
 #pragma unroll
         for (int l = 0; l < LR; ++l) {
             if (l >= l0) {
-                const float sx = row_sum16(rp[l].x), sy = row_sum16(rp[l].y);
+                const R sx = wave::row_sum16(rp[l].x), sy = wave::row_sum16(rp[l].y);
                 if (c == 0) RP[r + PR * l] = mk(sx, sy);
             }
         }
@@ -318,16 +331,16 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         // (Re, Im) of block column m (q < 2) or m + 1 (q >= 2)
         {
             const int q = lane >> 4;
-            float* cpw = &CP[wv][0].x;
+            R* cpw = &CP[wv][0].x;
 #pragma unroll
             for (int m = 0; m < LC; m += 2) {
                 if (m + 1 >= m0) {
-                    const float t = cross_rows4_pair(cp[m].x, cp[m].y, cp[m + 1].x, cp[m + 1].y);
+                    const R t = cross_rows4_pair(cp[m].x, cp[m].y, cp[m + 1].x, cp[m + 1].y);
                     cpw[2 * (c + PC * (m + (q >> 1))) + (q & 1)] = t;
                 }
             }
         }
-        qv = wave_sum(qv);
+        qv = wave::sum(qv);
         if (lane == 0) Q[wv] = qv;
         __syncthreads();
         // ---- d. y = row sums + column sums of the waves, one row per lane ----
@@ -342,23 +355,23 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         }
         __syncthreads();
         // ---- e. w = tau y + alpha v, alpha = -(tau/2) p^H v = -(1/2)|tau|^2 v^H A v; rank-2 update ----
-        float vav = 0.f;
+        R vav = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) vav += Q[w];
-        const float alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
+        const R alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
         f2 wr[LR], wc[LC];
 #pragma unroll
         for (int l = 0; l < LR; ++l) {
             f2 w = mul(tau, Y[r + PR * l]);
-            w.x = fmaf(alpha, vr[l].x, w.x);
-            w.y = fmaf(alpha, vr[l].y, w.y);
+            w.x = fma(alpha, vr[l].x, w.x);
+            w.y = fma(alpha, vr[l].y, w.y);
             wr[l] = w;
         }
 #pragma unroll
         for (int m = 0; m < LC; ++m) {
             f2 w = mul(tau, Y[c + PC * m]);
-            w.x = fmaf(alpha, vc[m].x, w.x);
-            w.y = fmaf(alpha, vc[m].y, w.y);
+            w.x = fma(alpha, vc[m].x, w.x);
+            w.y = fma(alpha, vc[m].y, w.y);
             wc[m] = w;
         }
 #pragma unroll
@@ -370,26 +383,26 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                     // straddle the diagonal
                     f2 vi = vr[l], wi = wr[l];
                     if (m >= Lay::mcross(l)) {
-                        const float h = HF[l][m - Lay::mcross(l)];
+                        const R h = HF[l][m - Lay::mcross(l)];
                         vi = mk(h * vi.x, h * vi.y);
                         wi = mk(h * wi.x, h * wi.y);
                     }
                     f2& a = A[Lay::idx(l, m)];
-                    a.x = fmaf(-vi.x, wc[m].x, a.x);
-                    a.x = fmaf(-vi.y, wc[m].y, a.x);
-                    a.x = fmaf(-wi.x, vc[m].x, a.x);
-                    a.x = fmaf(-wi.y, vc[m].y, a.x);
-                    a.y = fmaf(-vi.y, wc[m].x, a.y);
-                    a.y = fmaf(vi.x, wc[m].y, a.y);
-                    a.y = fmaf(-wi.y, vc[m].x, a.y);
-                    a.y = fmaf(wi.x, vc[m].y, a.y);
+                    a.x = fma(-vi.x, wc[m].x, a.x);
+                    a.x = fma(-vi.y, wc[m].y, a.x);
+                    a.x = fma(-wi.x, vc[m].x, a.x);
+                    a.x = fma(-wi.y, vc[m].y, a.x);
+                    a.y = fma(-vi.y, wc[m].x, a.y);
+                    a.y = fma(vi.x, wc[m].y, a.y);
+                    a.y = fma(-wi.y, vc[m].x, a.y);
+                    a.y = fma(wi.x, vc[m].y, a.y);
                 }
             }
         }
     }
     if (s_stop < n - 1) {
         // hand the trailing block A[s_stop:, s_stop:] over (diagonal stored halved)
-        float2* B = Bout + kk * (int64_t)ldb * ldb;
+        VT* B = Bout + kk * (int64_t)ldb * ldb;
 #pragma unroll
         for (int l = 0; l < LR; ++l)
 #pragma unroll
@@ -397,8 +410,8 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                 const int i = r + PR * l, j = c + PC * m;
                 if (i < n && j >= s_stop && i >= j) {
                     const f2 a = A[Lay::idx(l, m)];
-                    const float f = (i == j) ? 2.0f : 1.0f;
-                    B[(i - s_stop) + (int64_t)(j - s_stop) * ldb] = make_float2(a.x * f, a.y * f);
+                    const R f = (i == j) ? R(2) : R(1);
+                    B[(i - s_stop) + (int64_t)(j - s_stop) * ldb] = store_c<VT>(a.x * f, a.y * f);
                 }
             }
         return;
@@ -415,7 +428,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                     if (m <= Lay::mmax(l) && c == cs && r + PR * l == s) d[s] = 2.0 * (double)A[Lay::idx(l, m)].x;
         if (tid == 0) {
             e[s] = 0.0;
-            tau_o[s] = make_float2(0.f, 0.f);
+            tau_o[s] = store_c<VT>(R(0), R(0));
         }
     }
 }
@@ -461,22 +474,22 @@ size_t tridiag_reg_work_bytes(int n) {
 }
 
 namespace {
-template <int LR, int LC, int MINWG, typename THin>
-void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, float2* V, int ldv, int64_t v_kstride,
-               double* dd, double* de, float2* dtau, int64_t de_kstride, int s_end, float2* B, int ldb, hipStream_t s) {
-    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, THin>), dim3((unsigned)nk), dim3(512), 0, s, H,
-                       h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
+template <typename R, typename VT, int LR, int LC, int MINWG, typename THin>
+void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, VT* V, int ldv, int64_t v_kstride,
+               double* dd, double* de, VT* dtau, int64_t de_kstride, int s_end, VT* B, int ldb, hipStream_t s) {
+    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, R, VT, THin>), dim3((unsigned)nk), dim3(512), 0, s,
+                       H, h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
 }
 
 template <typename THin>
 void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk,
                 double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
     switch (g) {
-        case 0: reg_stage<6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 1: reg_stage<5, 10, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 2: reg_stage<4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<3, 6, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        default: reg_stage<2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 0: reg_stage<float, float2, 6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<float, float2, 5, 10, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 2: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<float, float2, 3, 6, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: reg_stage<float, float2, 2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
 }  // namespace
@@ -505,6 +518,79 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
                    dtau + off, n, t ? m - t : m, t ? buf[cur ^ 1] : (float2*)nullptr, t, s);
         if (!t) break;
         m = t;
+        cur ^= 1;
+    }
+    return hipGetLastError();
+}
+
+// ---- FP64: grids 128 (4 x 8, 250 VGPRs), 96 (3 x 6), 64 (2 x 4, two workgroups
+// per CU).  A larger FP64 matrix runs the blocked reduction first and hands
+// over its trailing block at a panel boundary.
+namespace {
+constexpr int kNumGrids64 = 3;
+constexpr int kGrid64[kNumGrids64] = {128, 96, 64};
+int grid64_of(int n) {
+    int g = 0;
+    while (g < kNumGrids64 - 1 && n <= kGrid64[g + 1]) ++g;
+    return g;
+}
+int split64(int n) {
+    const int g = grid64_of(n);
+    return g < kNumGrids64 - 1 ? kGrid64[g + 1] : 0;
+}
+bool f64_reg_enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("UEPM_F64_REG");
+        return !(v && v[0] == '0');
+    }();
+    return on;
+}
+void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv, int64_t vk,
+                  double* dd, double* de, double2* dtau, int64_t dk, int s_end, double2* B, int ldb, hipStream_t s) {
+    switch (g) {
+        case 0: reg_stage<double, double2, 4, 8, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<double, double2, 3, 6, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: reg_stage<double, double2, 2, 4, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+    }
+}
+}  // namespace
+
+int tridiag_reg_c128_tail(int n) {
+    if (!f64_reg_enabled() || n < 2) return 0;
+    if (n <= kGrid64[0]) return n;
+    // the blocked reduction stops at a panel boundary s0 >= n - 128
+    const int P = tridiag_blocked_panel();
+    const int s0 = ((n - kGrid64[0] + P - 1) / P) * P;
+    return n - s0;
+}
+
+size_t tridiag_reg_c128_work_bytes(int m) {
+    const int t1 = split64(m);
+    if (!t1) return 0;
+    const int t2 = split64(t1);
+    return sizeof(double2) * ((size_t)t1 * t1 + (size_t)t2 * t2);
+}
+
+hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv,
+                                   int64_t vk, double* dd, double* de, double2* dtau, int64_t dk, double2* work,
+                                   hipStream_t s) {
+    if (m < 1 || m > kGrid64[0] || nk <= 0) return hipErrorInvalidValue;
+    const int t1 = split64(m);
+    if (!t1) {
+        grid64_stage(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m, (double2*)nullptr, 0, s);
+        return hipGetLastError();
+    }
+    if (!work) return hipErrorInvalidValue;
+    double2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
+    grid64_stage(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m - t1, buf[0], t1, s);
+    int r = t1, cur = 0;
+    while (true) {
+        const int off = m - r;
+        const int t = split64(r);
+        grid64_stage(grid64_of(r), buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv, vk, dd + off,
+                     de + off, dtau + off, dk, t ? r - t : r, t ? buf[cur ^ 1] : (double2*)nullptr, t, s);
+        if (!t) break;
+        r = t;
         cur ^= 1;
     }
     return hipGetLastError();

@@ -75,7 +75,10 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
         z.X0 = sizeof(double2) * (size_t)n * nbr * chunk;
         if (nbr > 16) z.Y = sizeof(double2) * (size_t)n * nbr * chunk;
     } else if (!p->tri_v1) {
-        z.S = sizeof(double2) * (size_t)p->tp.scratch_elems * chunk;
+        // blocked-reduction scratch, then (after it) the FP64 register grids' hand-over blocks
+        const int t = uepm::tridiag_reg_c128_tail(n);
+        const size_t blocked = t < n ? sizeof(double2) * (size_t)p->tp.scratch_elems : 0;
+        z.S = std::max(blocked, uepm::tridiag_reg_c128_work_bytes(t)) * chunk;
     }
     if (stage_vec) {
         z.Z = sizeof(double) * (size_t)n * nbr * chunk;
@@ -558,7 +561,10 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     };
     bool used[kNumStages] = {true, true, true, stage_vec, mixed};
     p->stage_name[kTridiag] = mixed ? (p->tri_reg ? "tridiag_reg_kernel<c64>" : "tridiag_blocked_kernel<c64>")
-                                    : (p->tri_v1 ? "tridiag_kernel<c128>" : "tridiag_blocked_kernel<c128>");
+                                    : p->tri_v1 ? "tridiag_kernel<c128>"
+                                    : uepm::tridiag_reg_c128_tail(n) == n ? "tridiag_reg_kernel<c128>"
+                                    : uepm::tridiag_reg_c128_tail(n) > 0 ? "tridiag_blocked+reg_kernel<c128>"
+                                                                         : "tridiag_blocked_kernel<c128>";
     p->stage_name[kTriEig] = "tri_eig_wave_kernel";
     p->stage_name[kBack] = p->bt_v1 ? "backtransform_kernel"
                                     : (mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>");
@@ -613,9 +619,23 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         HIP_OK(rec(ci, kTridiag, 0));
         if (p->tri_v1)
             HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
-        else
-            HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, p->S.p, Hc, p->d.as<double>(), p->e.as<double>(),
-                                                    p->tau.as<double2>(), s));
+        else {
+            // the blocked reduction for the leading steps, the FP64 register grids for the
+            // trailing block of size t (the whole matrix when n <= 128); reflectors in place in H
+            const int t = uepm::tridiag_reg_c128_tail(n);
+            const int s0 = n - t;
+            const int64_t nn = (int64_t)n * n;
+            if (t < n)
+                HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, p->S.p, Hc, p->d.as<double>(),
+                                                        p->e.as<double>(), p->tau.as<double2>(), s,
+                                                        t > 0 ? s0 : n - 1));
+            if (t > 0) {
+                double2* B = Hc + s0 + (int64_t)s0 * n;
+                HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, p->d.as<double>() + s0,
+                                                     p->e.as<double>() + s0, p->tau.as<double2>() + s0, n,
+                                                     p->S.as<double2>(), s));
+            }
+        }
         HIP_OK(rec(ci, kTridiag, 1));
         HIP_OK(rec(ci, kTriEig, 0));
         HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nb, c, devals + c0 * nb,
