@@ -27,7 +27,8 @@ hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int 
                                       double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s,
                                       int s_end = 1 << 30);
 hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                      float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s);
+                                      float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s,
+                                      int s_end = 1 << 30);
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s);
@@ -66,6 +67,15 @@ int tridiag_reg_split(int n);
 size_t tridiag_reg_work_bytes(int n);
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s);
+// Complex64 register grids after the blocked reduction for n > 192: the
+// trailing size they take over (a panel boundary of the blocked kernel; n
+// itself for n <= 192; 0 with UEPM_REG_SPLIT=0) and the launch on a complex64
+// block left in place by the blocked kernel (arguments as for the FP64 form
+// below; the hand-over workspace is tridiag_reg_work_bytes(m)).
+int tridiag_reg_c64_tail(int n);
+hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv,
+                                        int64_t vk, double* dd, double* de, float2* dtau, int64_t dk, float2* work,
+                                        hipStream_t s);
 // The same register-resident reduction in FP64 (grids 128, 96, 64): trailing
 // size it takes over for a matrix of size n (n itself when n <= 128; 0 with
 // UEPM_F64_REG=0), its hand-over workspace per k-point (bytes), and the

@@ -616,8 +616,8 @@ hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int 
 }
 
 hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
-                                      float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s) {
-    return launch_tri_blocked_t<float>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, tp.n - 1, s);
+                                      float2* Vout, double* dd, double* de, float2* dtau, hipStream_t s, int s_end) {
+    return launch_tri_blocked_t<float>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
 }
 
 }  // namespace uepm

@@ -494,33 +494,59 @@ void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, flo
 }
 }  // namespace
 
+namespace {
+// The complex64 chain on a block of size m whose element (0, 0) sits at H
+// (leading dimension ld, per-k stride hk); outputs at the block's offset of
+// the full matrix's V (ldv, stride vk) and d, e, tau (stride dk).
+template <typename THin>
+hipError_t c64_chain(const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk, double* dd,
+                     double* de, float2* dtau, int64_t dk, float2* work, hipStream_t s) {
+    const int t1 = tridiag_reg_split(m);
+    if (!t1) {
+        grid_stage(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m, (float2*)nullptr, 0, s);
+        return hipGetLastError();
+    }
+    if (!work) return hipErrorInvalidValue;
+    // first grid on the block, then each smaller grid on the trailing block of
+    // its size, in ping-pong blocks (the block of size t lives at
+    // buf[i] + k * t * t; a stage reads all of its block before it writes the next)
+    float2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
+    grid_stage(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m - t1, buf[0], t1, s);
+    int r = t1, cur = 0;
+    while (true) {
+        const int off = m - r;
+        const int t = tridiag_reg_split(r);
+        grid_stage(grid_of(r), (const float2*)buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv,
+                   vk, dd + off, de + off, dtau + off, dk, t ? r - t : r, t ? buf[cur ^ 1] : (float2*)nullptr, t,
+                   s);
+        if (!t) break;
+        r = t;
+        cur ^= 1;
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s) {
     if (n < 1 || n > 192 || nk <= 0) return hipErrorInvalidValue;
     const int64_t hk = (int64_t)ld * n;
-    const int t1 = tridiag_reg_split(n);
-    if (!t1) {
-        grid_stage(grid_of(n), dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n, (float2*)nullptr, 0, s);
-        return hipGetLastError();
-    }
-    if (!work) return hipErrorInvalidValue;
-    // first grid on the full matrix, then each smaller grid on the trailing block
-    // of its size, in ping-pong blocks (the block of size t lives at
-    // buf[i] + k * t * t; a stage reads all of its block before it writes the next)
-    float2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
-    grid_stage(grid_of(n), dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, n - t1, buf[0], t1, s);
-    int m = t1, cur = 0;
-    while (true) {
-        const int off = n - m;
-        const int t = tridiag_reg_split(m);
-        float2* Vo = Vout + off + (int64_t)off * ld;
-        grid_stage(grid_of(m), (const float2*)buf[cur], (int64_t)m * m, m, m, nk, Vo, ld, hk, dd + off, de + off,
-                   dtau + off, n, t ? m - t : m, t ? buf[cur ^ 1] : (float2*)nullptr, t, s);
-        if (!t) break;
-        m = t;
-        cur ^= 1;
-    }
-    return hipGetLastError();
+    return c64_chain(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, work, s);
+}
+
+int tridiag_reg_c64_tail(int n) {
+    if (reg_split_mode() == 0 || n < 2) return 0;
+    if (n <= kGrid[0]) return n;
+    const int P = tridiag_blocked_panel();
+    const int s0 = ((n - kGrid[0] + P - 1) / P) * P;
+    return n - s0;
+}
+
+hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv,
+                                        int64_t vk, double* dd, double* de, float2* dtau, int64_t dk, float2* work,
+                                        hipStream_t s) {
+    if (m < 1 || m > kGrid[0] || nk <= 0) return hipErrorInvalidValue;
+    return c64_chain(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, work, s);
 }
 
 // ---- FP64: grids 128 (4 x 8, 250 VGPRs), 96 (3 x 6), 64 (2 x 4, two workgroups

@@ -53,6 +53,13 @@ int64_t plan_chunk(const uepm_plan* p, int64_t nk) {
 }
 
 namespace {
+// Trailing size of the precision form's blocked reduction (n > 192) that the
+// register grids take over; 0 when the configuration asks for the blocked kernel.
+int mixed_c64_tail(const uepm_plan* p) {
+    if (p->tri_reg || p->cfg.tridiag == 1 || p->dim <= uepm::tridiag_reg_max_n()) return 0;
+    return uepm::tridiag_reg_c64_tail(p->dim);
+}
+
 struct Sizes {
     size_t H, d, e, tau, Vf, tauf, S32, Ev, X0, Y, S, Z, W, X;
 };
@@ -70,7 +77,9 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
         z.Vf = sizeof(float2) * (size_t)n * n * chunk;
         z.tauf = sizeof(float2) * (size_t)n * chunk;
         z.S32 = p->tri_reg ? uepm::tridiag_reg_work_bytes(n) * chunk
-                           : sizeof(float2) * (size_t)p->tp32.scratch_elems * chunk;
+                           : std::max(sizeof(float2) * (size_t)p->tp32.scratch_elems,
+                                      uepm::tridiag_reg_work_bytes(mixed_c64_tail(p))) *
+                                 chunk;
         z.Ev = sizeof(double) * (size_t)nbr * chunk;
         z.X0 = sizeof(double2) * (size_t)n * nbr * chunk;
         if (nbr > 16) z.Y = sizeof(double2) * (size_t)n * nbr * chunk;
@@ -560,7 +569,9 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end], s);
     };
     bool used[kNumStages] = {true, true, true, stage_vec, mixed};
-    p->stage_name[kTridiag] = mixed ? (p->tri_reg ? "tridiag_reg_kernel<c64>" : "tridiag_blocked_kernel<c64>")
+    p->stage_name[kTridiag] = mixed ? (p->tri_reg              ? "tridiag_reg_kernel<c64>"
+                                       : mixed_c64_tail(p) > 0 ? "tridiag_blocked+reg_kernel<c64>"
+                                                               : "tridiag_blocked_kernel<c64>")
                                     : p->tri_v1 ? "tridiag_kernel<c128>"
                                     : uepm::tridiag_reg_c128_tail(n) == n ? "tridiag_reg_kernel<c128>"
                                     : uepm::tridiag_reg_c128_tail(n) > 0 ? "tridiag_blocked+reg_kernel<c128>"
@@ -585,9 +596,22 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             if (p->tri_reg)
                 HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, p->d.as<double>(), p->e.as<double>(),
                                                     p->tauf.as<float2>(), p->S32.as<float2>(), s));
-            else
+            else {
+                // n > 192: the blocked reduction for the leading steps, the register grids
+                // for the trailing block it leaves in place in Vc
+                const int t = mixed_c64_tail(p);
+                const int s0 = n - t;
                 HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
-                                                        p->e.as<double>(), p->tauf.as<float2>(), s));
+                                                        p->e.as<double>(), p->tauf.as<float2>(), s,
+                                                        t > 0 ? s0 : n - 1));
+                if (t > 0) {
+                    const int64_t nn = (int64_t)n * n;
+                    float2* B = Vc + s0 + (int64_t)s0 * n;
+                    HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, p->d.as<double>() + s0,
+                                                              p->e.as<double>() + s0, p->tauf.as<float2>() + s0, n,
+                                                              p->S32.as<float2>(), s));
+                }
+            }
             HIP_OK(rec(ci, kTridiag, 1));
             HIP_OK(rec(ci, kTriEig, 0));
             // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
