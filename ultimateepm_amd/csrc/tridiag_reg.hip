@@ -159,7 +159,7 @@ struct RegLayout {
 // ldb, ldb * ldb per k) for a smaller kernel to continue.  Input (leading
 // dimension ld), reflectors (ldv) and d, e, tau of k-point kk start at
 // kk * h_kstride, kk * v_kstride and kk * de_kstride.
-template <int PR, int PC, int LR, int LC, int MINWG, typename R, typename VT, typename THin>
+template <int PR, int PC, int LR, int LC, int MINWG, typename R, typename VT, typename THin, bool LOW = false>
 __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* __restrict__ Hin, int64_t h_kstride,
                                                                     int n, int ld, VT* __restrict__ Vout,
                                                                     int ldv, int64_t v_kstride,
@@ -178,7 +178,10 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     static_assert(PC == 16 && BS % 64 == 0, "row groups are the 16-lane DPP rows of a wave");
     static_assert(PC * LC == NX && NX <= BS && LC % 2 == 0, "square coverage, one y entry per lane");
 
-    __shared__ f2 X[NX];       // column s: zero at rows <= s and >= n
+    // column s: zero at rows <= s and >= n; two buffers by step parity, since the
+    // LOW variant still reads the column of step s in its update phase while a
+    // wave ahead may write the column of step s + 1 (no barrier in between)
+    __shared__ f2 Xbuf[2][NX];
     __shared__ f2 XD[NX];      // column s as stored (its diagonal entry is A(s, s) / 2)
     __shared__ f2 RP[NX];      // row sums of the symmetric product
     __shared__ f2 CP[NW][NX];  // column sums, one row per wave
@@ -224,6 +227,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     const f2 one = mk<R>(1, 0), zero = mk<R>(0, 0);
 
     for (int s = 0; s < s_stop; ++s) {
+        f2* X = Xbuf[s & 1];
         const int cs = s % PC, ms = s / PC;
         const int l0 = (s + 1) / PR, m0 = (s + 1) / PC;  // first live row / column blocks
         // ---- a. column s to LDS (every row of the grid) ----
@@ -285,33 +289,55 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         }
         // v at this lane's rows and columns (X is zero outside s+1..n-1; loads
         // unconditional so they issue together)
-        f2 vr[LR], vc[LC];
+        // LOW (fewer registers, for two workgroups per CU): the column values of v and
+        // w are formed from LDS where they are used instead of being kept, and each
+        // pair of block columns is reduced as soon as its partial sums are complete
+        // (the same operations in the same order)
+        auto vcol = [&](int m) -> f2 {
+            const int j = c + PC * m;
+            return sel(j == s + 1, one, mul(X[j], sc));
+        };
+        f2 vr[LR], vc[LOW ? 1 : LC];
 #pragma unroll
         for (int l = 0; l < LR; ++l) {
             const int i = r + PR * l;
             const f2 v = mul(X[i], sc);
             vr[l] = sel(i == s + 1, one, v);
         }
+        if constexpr (!LOW) {
 #pragma unroll
-        for (int m = 0; m < LC; ++m) {
-            const int j = c + PC * m;
-            const f2 v = mul(X[j], sc);
-            vc[m] = sel(j == s + 1, one, v);
+            for (int m = 0; m < LC; ++m) vc[m] = vcol(m);
         }
         // ---- c. y = A v on the live blocks; local share of v^H A v ----
-        f2 rp[LR], cp[LC];
+        f2 rp[LR], cp[LOW ? 2 : LC];
 #pragma unroll
         for (int l = 0; l < LR; ++l) rp[l] = zero;
 #pragma unroll
-        for (int m = 0; m < LC; ++m) cp[m] = zero;
+        for (int m = 0; m < (LOW ? 2 : LC); ++m) cp[m] = zero;
+        const int q4 = lane >> 4;
+        R* cpw = &CP[wv][0].x;
 #pragma unroll
         for (int m = LC - 1; m >= 0; --m) {
+            const int mc = LOW ? (m & 1) : m;  // cp slot of block column m
             if (m >= m0) {  // a stored block is live while its block column is
+                const f2 vcm = LOW ? vcol(m) : vc[LOW ? 0 : m];
 #pragma unroll
                 for (int l = Lay::lmin(m); l < LR; ++l) {
                     const f2 a = A[Lay::idx(l, m)];
-                    fma_ab(rp[l], a, vc[m]);
-                    fma_cab(cp[m], a, vr[l]);
+                    fma_ab(rp[l], a, vcm);
+                    fma_cab(cp[mc], a, vr[l]);
+                }
+            }
+            if constexpr (LOW) {
+                // column sums of the pair (m, m + 1), m even: lane row q holds (Re, Im)
+                // of block column m (q < 2) or m + 1 (q >= 2)
+                if ((m & 1) == 0) {
+                    if (m + 1 >= m0) {
+                        const R t = cross_rows4_pair(cp[0].x, cp[0].y, cp[1].x, cp[1].y);
+                        cpw[2 * (c + PC * (m + (q4 >> 1))) + (q4 & 1)] = t;
+                    }
+                    cp[0] = zero;
+                    cp[1] = zero;
                 }
             }
         }
@@ -329,14 +355,12 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         }
         // column sums, two block columns per reduce-scatter: lane row q holds
         // (Re, Im) of block column m (q < 2) or m + 1 (q >= 2)
-        {
-            const int q = lane >> 4;
-            R* cpw = &CP[wv][0].x;
+        if constexpr (!LOW) {
 #pragma unroll
             for (int m = 0; m < LC; m += 2) {
                 if (m + 1 >= m0) {
-                    const R t = cross_rows4_pair(cp[m].x, cp[m].y, cp[m + 1].x, cp[m + 1].y);
-                    cpw[2 * (c + PC * (m + (q >> 1))) + (q & 1)] = t;
+                    const R t = cross_rows4_pair(cp[m].x, cp[m].y, cp[LOW ? 0 : m + 1].x, cp[LOW ? 0 : m + 1].y);
+                    cpw[2 * (c + PC * (m + (q4 >> 1))) + (q4 & 1)] = t;
                 }
             }
         }
@@ -359,7 +383,13 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
 #pragma unroll
         for (int w = 0; w < NW; ++w) vav += Q[w];
         const R alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
-        f2 wr[LR], wc[LC];
+        auto wcol = [&](int m, f2 v) -> f2 {
+            f2 w = mul(tau, Y[c + PC * m]);
+            w.x = fma(alpha, v.x, w.x);
+            w.y = fma(alpha, v.y, w.y);
+            return w;
+        };
+        f2 wr[LR], wc[LOW ? 1 : LC];
 #pragma unroll
         for (int l = 0; l < LR; ++l) {
             f2 w = mul(tau, Y[r + PR * l]);
@@ -367,16 +397,15 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
             w.y = fma(alpha, vr[l].y, w.y);
             wr[l] = w;
         }
+        if constexpr (!LOW) {
 #pragma unroll
-        for (int m = 0; m < LC; ++m) {
-            f2 w = mul(tau, Y[c + PC * m]);
-            w.x = fma(alpha, vc[m].x, w.x);
-            w.y = fma(alpha, vc[m].y, w.y);
-            wc[m] = w;
+            for (int m = 0; m < LC; ++m) wc[m] = wcol(m, vc[m]);
         }
 #pragma unroll
         for (int m = LC - 1; m >= 0; --m) {
             if (m >= m0) {
+                const f2 vcm = LOW ? vcol(m) : vc[LOW ? 0 : m];
+                const f2 wcm = LOW ? wcol(m, vcm) : wc[LOW ? 0 : m];
 #pragma unroll
                 for (int l = Lay::lmin(m); l < LR; ++l) {
                     // a -= v_i conj(w_j) + w_i conj(v_j), scaled by 1, 1/2 or 0 on blocks that
@@ -388,14 +417,14 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                         wi = mk(h * wi.x, h * wi.y);
                     }
                     f2& a = A[Lay::idx(l, m)];
-                    a.x = fma(-vi.x, wc[m].x, a.x);
-                    a.x = fma(-vi.y, wc[m].y, a.x);
-                    a.x = fma(-wi.x, vc[m].x, a.x);
-                    a.x = fma(-wi.y, vc[m].y, a.x);
-                    a.y = fma(-vi.y, wc[m].x, a.y);
-                    a.y = fma(vi.x, wc[m].y, a.y);
-                    a.y = fma(-wi.y, vc[m].x, a.y);
-                    a.y = fma(wi.x, vc[m].y, a.y);
+                    a.x = fma(-vi.x, wcm.x, a.x);
+                    a.x = fma(-vi.y, wcm.y, a.x);
+                    a.x = fma(-wi.x, vcm.x, a.x);
+                    a.x = fma(-wi.y, vcm.y, a.x);
+                    a.y = fma(-vi.y, wcm.x, a.y);
+                    a.y = fma(vi.x, wcm.y, a.y);
+                    a.y = fma(-wi.y, vcm.x, a.y);
+                    a.y = fma(wi.x, vcm.y, a.y);
                 }
             }
         }
@@ -474,11 +503,11 @@ size_t tridiag_reg_work_bytes(int n) {
 }
 
 namespace {
-template <typename R, typename VT, int LR, int LC, int MINWG, typename THin>
+template <typename R, typename VT, int LR, int LC, int MINWG, typename THin, bool LOW = false>
 void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, VT* V, int ldv, int64_t v_kstride,
                double* dd, double* de, VT* dtau, int64_t de_kstride, int s_end, VT* B, int ldb, hipStream_t s) {
-    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, R, VT, THin>), dim3((unsigned)nk), dim3(512), 0, s,
-                       H, h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
+    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, R, VT, THin, LOW>), dim3((unsigned)nk), dim3(512), 0,
+                       s, H, h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
 }
 
 template <typename THin>
@@ -486,7 +515,7 @@ void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, flo
                 double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
     switch (g) {
         case 0: reg_stage<float, float2, 6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 1: reg_stage<float, float2, 5, 10, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 2: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 3: reg_stage<float, float2, 3, 6, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         default: reg_stage<float, float2, 2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
