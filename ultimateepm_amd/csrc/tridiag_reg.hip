@@ -517,8 +517,8 @@ void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, flo
         case 0: reg_stage<float, float2, 6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 1: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 2: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<float, float2, 3, 6, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        default: reg_stage<float, float2, 2, 4, 2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<float, float2, 3, 6, 6, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: reg_stage<float, float2, 2, 4, 8, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
 }  // namespace
