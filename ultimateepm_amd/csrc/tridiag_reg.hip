@@ -215,14 +215,14 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         }
     }
     // update weights of the blocks that straddle the diagonal: 1 below, 1/2 on, 0 above
-    R HF[LR][XC];
+    R HF[LOW ? 1 : LR][LOW ? 1 : XC];  // LOW recomputes the weights where used
 #pragma unroll
     for (int l = 0; l < LR; ++l)
 #pragma unroll
         for (int q = 0; q < XC; ++q) {
             const int m = Lay::mcross(l) + q;
             const int dd = (r - c) + PR * l - PC * m;
-            HF[l][q] = dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0));
+            if constexpr (!LOW) HF[l][q] = dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0));
         }
     const f2 one = mk<R>(1, 0), zero = mk<R>(0, 0);
 
@@ -412,7 +412,9 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                     // straddle the diagonal
                     f2 vi = vr[l], wi = wr[l];
                     if (m >= Lay::mcross(l)) {
-                        const R h = HF[l][m - Lay::mcross(l)];
+                        const int dd = (r - c) + PR * l - PC * m;
+                        const R h = LOW ? (dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0)))
+                                        : HF[LOW ? 0 : l][LOW ? 0 : m - Lay::mcross(l)];
                         vi = mk(h * vi.x, h * vi.y);
                         wi = mk(h * wi.x, h * wi.y);
                     }
@@ -514,7 +516,7 @@ template <typename THin>
 void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk,
                 double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
     switch (g) {
-        case 0: reg_stage<float, float2, 6, 12, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 0: reg_stage<float, float2, 6, 12, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 1: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 2: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 3: reg_stage<float, float2, 3, 6, 6, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
