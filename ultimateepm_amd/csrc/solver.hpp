@@ -76,8 +76,8 @@ int tridiag_reg_c64_tail(int n);
 hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv,
                                         int64_t vk, double* dd, double* de, float2* dtau, int64_t dk, float2* work,
                                         hipStream_t s);
-// The same register-resident reduction in FP64 (grids 128, 96, 64): trailing
-// size it takes over for a matrix of size n (n itself when n <= 128; 0 with
+// The same register-resident reduction in FP64 (grids 192 .. 64): trailing
+// size it takes over for a matrix of size n (n itself when n <= 192; 0 with
 // UEPM_F64_REG=0), its hand-over workspace per k-point (bytes), and the
 // launch on a block of size m whose element (0, 0) sits at H (leading
 // dimension ld, per-k stride hk); reflectors, d, e, tau at the block's offset
