@@ -14,7 +14,10 @@
 // reference only through the last-ulp behaviour of sin/cos/exp/sqrt and of
 // the Bessel functions: those are evaluated here in closed form (with power
 // series where the closed form cancels) instead of libstdc++'s
-// std::cyl_bessel_j(l + 1/2, x), which the reference calls per element.
+// std::cyl_bessel_j(l + 1/2, x), which the reference calls per element.  The
+// one rearrangement is the non-local phase exp(2 pi i (tau/a) . (K_i - K_j)),
+// formed as exp(i theta_i) exp(-i theta_j) from per-vector values (a few ulp,
+// the size of the rounding of K_i - K_j itself).
 //
 // The per-basis-vector quantities (|K_i|, j_l(|K_i| R), SOC radial factors)
 // are k-dependent but element-independent, so they are computed once per
@@ -146,6 +149,8 @@ struct KState {
     double nK;           // |K|
     BesselTriple b0a, b0c, b2a, b2c;  // l = 0 / l = 2 triples at R0 / R2 of anion / cation
     double Ba, Bc;       // SOC radial factors B(|K|) of anion / cation
+    double pc, ps;       // cos / sin of 2 pi (tau/a) (kx + ky + kz): the non-local phase of a
+                         // pair (i, j) is exp(i (theta_i - theta_j))
 };
 
 // SpinOrbitFunctional.cpp:20-49 (period 2, 3, 4; others rejected on the host)
@@ -166,6 +171,9 @@ __device__ __forceinline__ void kstate_init(const DevParams& P, const double* k,
     s.Kz = s.kz * P.fourier;
     s.nK = sqrt(s.Kx * s.Kx + s.Ky * s.Ky + s.Kz * s.Kz);
     if (P.nl) {
+        sincos(2.0 * 3.14159265358979323846 *
+                   (P.tau_over_a * s.kx + P.tau_over_a * s.ky + P.tau_over_a * s.kz),
+               &s.ps, &s.pc);
         if (P.need_l0_a) s.b0a = bessel_triple(0, s.nK, P.R0_a);
         if (P.need_l0_c) s.b0c = bessel_triple(0, s.nK, P.R0_c);
         if (P.well == 0) {
@@ -226,9 +234,11 @@ __device__ __forceinline__ void nonlocal(const DevParams& P, const KState& a, co
     }
     const double Vs = 1.0 * (Va + Vc) / 2.0;
     const double Vas = 1.0 * (Va - Vc) / 2.0;
-    const double Gtau = P.tau_over_a * (a.kx - b.kx) + P.tau_over_a * (a.ky - b.ky) + P.tau_over_a * (a.kz - b.kz);
-    double s, c;
-    sincos(2.0 * 3.14159265358979323846 * Gtau, &s, &c);
+    // cos / sin of 2 pi (tau/a) sum_c (K_a - K_b)_c from the per-vector phases
+    // (the reference forms the difference first; the two agree to rounding, and
+    // the product form keeps the (b, a) value the exact conjugate)
+    const double c = a.pc * b.pc + a.ps * b.ps;
+    const double s = a.ps * b.pc - a.pc * b.ps;
     re = c * Vs;
     im = s * Vas;
 }

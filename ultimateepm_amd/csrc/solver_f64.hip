@@ -73,9 +73,11 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
     __syncthreads();
     double2* A = H + (size_t)kk * ld * P.dim;
     const int npairs = N * (N + 1) / 2;
-    // h_ji from h_ij: the phase arguments of V(G) and of the non-local term
-    // negate exactly under i <-> j, the device sincos is exactly odd / even
-    // there, and the square-well radial integrals are symmetric in |K_i|,
+    // h_ji from h_ij: the phase argument of V(G) negates exactly under
+    // i <-> j and the device sincos is exactly odd / even there; the
+    // non-local phase is cos / sin of theta_i - theta_j formed from the
+    // per-vector cos / sin, which swap to the exact conjugate (products
+    // commute, no FMA contraction); the square-well radial integrals are symmetric in |K_i|,
     // |K_j| outside the near-equal branch of F_l (which reads one side's
     // Bessel values) -- so one evaluation gives the reference's two-sided
     // 0.5 (h_ij + conj(h_ji)) bit for bit.  Gaussian l = 2 wells and
@@ -577,8 +579,13 @@ constexpr int kBtBS = 256;
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s) {
+    // one KState per basis vector in LDS: up to 890 vectors in the 160 KB of a CU
     const size_t shm = sizeof(KState) * (size_t)P.N;
-    hipLaunchKernelGGL(assemble_kernel<kAsmBS>, dim3((unsigned)nk), dim3(kAsmBS), shm, s, P, dG, dk, dH, ld);
+    if (shm > 163840) return hipErrorInvalidValue;
+    auto kern = assemble_kernel<kAsmBS>;
+    hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+    if (err != hipSuccess) return err;
+    hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(kAsmBS), shm, s, P, dG, dk, dH, ld);
     return hipGetLastError();
 }
 
