@@ -1,6 +1,7 @@
-// tridiag_reg.hip — register-resident complex64 Householder reduction of a
-// Hermitian H(k) to real symmetric tridiagonal form, one 512-lane workgroup
-// per k-point, for dim <= 192 (gfx950).
+// tridiag_reg.hip — register-resident Householder reduction of a Hermitian
+// H(k) to real symmetric tridiagonal form, one 512-lane workgroup per
+// k-point, for dim <= 192 (gfx950): complex64 arithmetic for the precision
+// form, complex128 for the FP64 form, and a chain of shrinking grids (below).
 //
 // Layout.  The 512 lanes form a PR x PC = 32 x 16 grid: lane (r, c) with
 // c = tid % 16, r = tid / 16 (a wave holds four r values and all sixteen c).
@@ -470,8 +471,10 @@ int tridiag_reg_max_n() { return 192; }
 // Hand-over between grids.  The late steps of a large grid carry its full
 // per-step overhead for a small trailing matrix, so each grid stops where the
 // next smaller one can take the trailing block: 192-grid (6 x 12 blocks) ->
-// 160 (5 x 10) -> 128 (4 x 8, 128 VGPRs: two workgroups per CU) -> 96 (3 x 6,
-// two per CU) -> 64 (2 x 4, three per CU).  UEPM_REG_SPLIT=0 keeps one grid.
+// 160 (5 x 10) -> 128 (4 x 8) -> 96 (3 x 6) -> 64 (2 x 4).  Register budgets
+// (workgroups per CU): 192 and 160 low-register, 128 VGPRs (2); 128 at 128
+// VGPRs (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
+// UEPM_REG_SPLIT=0 keeps one grid.
 namespace {
 int reg_split_mode() {
     static const int mode = [] {
