@@ -41,6 +41,7 @@ struct DevParams {
     int32_t need_l0_a, need_l0_c;  // F_0 needed (R0 != 0) for anion / cation
     int32_t need_l2_a, need_l2_c;  // A2 != 0
     int32_t period_a, period_c;    // SOC radial-factor periods
+    int32_t same_r0;    // need_l0_a && need_l0_c && R0_a == R0_c
     double a;           // lattice constant [m]
     double tau;         // a / 8 [m]
     double tpa;         // 2 pi / a
@@ -175,7 +176,7 @@ __device__ __forceinline__ void kstate_init(const DevParams& P, const double* k,
                    (P.tau_over_a * s.kx + P.tau_over_a * s.ky + P.tau_over_a * s.kz),
                &s.ps, &s.pc);
         if (P.need_l0_a) s.b0a = bessel_triple(0, s.nK, P.R0_a);
-        if (P.need_l0_c) s.b0c = bessel_triple(0, s.nK, P.R0_c);
+        if (P.need_l0_c) s.b0c = P.same_r0 ? s.b0a : bessel_triple(0, s.nK, P.R0_c);
         if (P.well == 0) {
             if (P.need_l2_a) s.b2a = bessel_triple(2, s.nK, P.R2_a);
             if (P.need_l2_c) s.b2c = bessel_triple(2, s.nK, P.R2_c);
@@ -191,8 +192,24 @@ __device__ __forceinline__ void kstate_init(const DevParams& P, const double* k,
     }
 }
 
-// Pseudopotential::GetValue for integer G = G_i - G_j (Pseudopotential.cpp:24-56).
-__device__ __forceinline__ void local_V(const DevParams& P, int gx, int gy, int gz, double& re, double& im) {
+// (cos, sin) of the phase of V(G) for integer G (Pseudopotential.cpp:24-56).
+__device__ __forceinline__ double2 local_phase(const DevParams& P, int gx, int gy, int gz) {
+    const double Gtau = P.tpa * (P.tau * (double)gx + P.tau * (double)gy + P.tau * (double)gz);
+    double s, c;
+    sincos(Gtau, &s, &c);
+    return make_double2(c, s);
+}
+
+// The form factors vanish outside G^2 in {3, 4, 8, 11}, which bounds every
+// component of G by 3: a table of local_phase over [-3, 3]^3 (343 entries,
+// built on the device with the same expression) serves every pair.
+constexpr int kVPhaseTab = 343;
+__device__ __forceinline__ int local_phase_index(int gx, int gy, int gz) { return (gx + 3) * 49 + (gy + 3) * 7 + (gz + 3); }
+
+// Pseudopotential::GetValue for integer G = G_i - G_j (Pseudopotential.cpp:24-56);
+// tab: nullptr to evaluate the phase here, else the table above.
+__device__ __forceinline__ void local_V(const DevParams& P, int gx, int gy, int gz, double& re, double& im,
+                                        const double2* tab = nullptr) {
     const int G2 = gx * gx + gy * gy + gz * gz;
     double VS = 0.0, VA = 0.0;
     switch (G2) {
@@ -202,30 +219,39 @@ __device__ __forceinline__ void local_V(const DevParams& P, int gx, int gy, int 
         case 11: VS = P.VS[3]; VA = P.VA[3]; break;
         default: re = 0.0; im = 0.0; return;  // VS = VA = 0: 0*cos = 0, 0*sin = 0
     }
-    const double Gtau = P.tpa * (P.tau * (double)gx + P.tau * (double)gy + P.tau * (double)gz);
-    double s, c;
-    sincos(Gtau, &s, &c);
-    re = VS * c;
-    im = VA * s;
+    const double2 cs = tab ? tab[local_phase_index(gx, gy, gz)] : local_phase(P, gx, gy, gz);
+    re = VS * cs.x;
+    im = VA * cs.y;
 }
 
 // Material::compute_pseudopotential_non_local_correction (Material.cpp:139-208)
 __device__ __forceinline__ void nonlocal(const DevParams& P, const KState& a, const KState& b, double& re, double& im) {
     const double n1 = a.nK, n2 = b.nK;
-    const double dp = a.Kx * b.Kx + a.Ky * b.Ky + a.Kz * b.Kz;
-    const double np = n1 * n2;
-    const double cosang = (np < 1.0e-13) ? 1.0 : dp / (n1 * n2);
-    const double leg2 = 0.5 * (3 * cosang * cosang - 1);
+    double leg2 = 0.0;  // P_2(cos angle(K1, K2)), used by the l = 2 terms only
+    if (P.need_l2_a || P.need_l2_c) {
+        const double dp = a.Kx * b.Kx + a.Ky * b.Ky + a.Kz * b.Kz;
+        const double np = n1 * n2;
+        const double cosang = (np < 1.0e-13) ? 1.0 : dp / (n1 * n2);
+        leg2 = 0.5 * (3 * cosang * cosang - 1);
+    }
     const double nn = n1 * n2 - P.kF2;
     double Va = 0.0, Vc = 0.0;
-    if (P.need_l0_a) Va = P.V_pre * (P.alpha0_a + P.dfbeta_a * nn) * F_l(n1, n2, P.R0_a, 0, a.b0a, b.b0a);
+    double F0a = 0.0;
+    if (P.need_l0_a) {
+        F0a = F_l(n1, n2, P.R0_a, 0, a.b0a, b.b0a);
+        Va = P.V_pre * (P.alpha0_a + P.dfbeta_a * nn) * F0a;
+    }
     if (P.need_l2_a) {
         double V2;
         if (P.well == 0) V2 = P.V_pre * P.A2_a * 5 * leg2 * F_l(n1, n2, P.R2_a, 2, a.b2a, b.b2a);
         else V2 = P.gpre_a * P.A2_a * leg2 * F_2_gauss(n1, n2, P.R2_a);
         Va += V2;
     }
-    if (P.need_l0_c) Vc = P.V_pre * (P.alpha0_c + P.dfbeta_c * nn) * F_l(n1, n2, P.R0_c, 0, a.b0c, b.b0c);
+    if (P.need_l0_c) {
+        // the same radius gives the same F_0 (the Bessel triples are equal too)
+        const double F0c = P.same_r0 ? F0a : F_l(n1, n2, P.R0_c, 0, a.b0c, b.b0c);
+        Vc = P.V_pre * (P.alpha0_c + P.dfbeta_c * nn) * F0c;
+    }
     if (P.need_l2_c) {
         double V2;
         if (P.well == 0) V2 = P.V_pre * P.A2_c * 5 * leg2 * F_l(n1, n2, P.R2_c, 2, a.b2c, b.b2c);

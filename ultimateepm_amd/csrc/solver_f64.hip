@@ -66,10 +66,12 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
                                                        int ld) {
     extern __shared__ __align__(16) unsigned char smem[];
     KState* ks = reinterpret_cast<KState*>(smem);
+    __shared__ double2 vtab[kVPhaseTab];
     const int kk = blockIdx.x;
     const double* k = kpts + 3 * (size_t)kk;
     const int N = P.N;
     for (int i = threadIdx.x; i < N; i += BS) kstate_init(P, k, G, i, ks[i]);
+    for (int t = threadIdx.x; t < kVPhaseTab; t += BS) vtab[t] = local_phase(P, t / 49 - 3, (t / 7) % 7 - 3, t % 7 - 3);
     __syncthreads();
     double2* A = H + (size_t)kk * ld * P.dim;
     const int npairs = N * (N + 1) / 2;
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
         const KState& sj = ks[j];
         const int gx = G[3 * i] - G[3 * j], gy = G[3 * i + 1] - G[3 * j + 1], gz = G[3 * i + 2] - G[3 * j + 2];
         double re_ij, im_ij;
-        local_V(P, gx, gy, gz, re_ij, im_ij);
+        local_V(P, gx, gy, gz, re_ij, im_ij, vtab);
         double re_ji = re_ij, im_ji = -im_ij;
         if (i == j) {
             const double KG2 = P.kin * (si.kx * si.kx + si.ky * si.ky + si.kz * si.kz);
@@ -320,7 +322,7 @@ __device__ __forceinline__ int sturm_count_u(const double* __restrict__ d, const
 // abstol_rel: bisection also stops once a bracket is narrower than
 // abstol_rel * ||T||_inf (0: to the rounding level of the eigenvalue itself).
 template <int S>
-__global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restrict__ dall,
+__global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __restrict__ dall,
                                                           const double* __restrict__ eall, int n, int nb, int64_t nk,
                                                           double* __restrict__ evals, double* __restrict__ Z,
                                                           double* __restrict__ W, int want_vectors,
@@ -385,11 +387,15 @@ __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restri
     // the nk * nb (k, band) pairs) hold only the U factor and the iterate.
     const int64_t T = nk * nb;
     const int64_t t = kk * nb + b;
-    double* U0 = W;                      // reciprocal pivots
-    double* U1 = W + (size_t)n * T;      // first super-diagonal
-    double* U2 = W + (size_t)2 * n * T;  // second super-diagonal
-    double* X = W + (size_t)3 * n * T;   // forward result, then iterate (unscaled)
+    double* __restrict__ U0 = W;                      // reciprocal pivots
+    double* __restrict__ U1 = W + (size_t)n * T;      // first super-diagonal
+    double* __restrict__ U2 = W + (size_t)2 * n * T;  // second super-diagonal
+    double* __restrict__ X = W + (size_t)3 * n * T;   // forward result, then iterate (unscaled)
 #define IDX(i) ((size_t)(i) * T + t)
+    // The loads of PB rows are issued together ahead of their use (each row is
+    // read before this sweep overwrites it), so a sweep waits on memory once
+    // per PB rows instead of once per row.
+    constexpr int PB = 4;
     const double tiny = eps * tnorm;
     double sc = 1.0;  // scaling of the iterate in X
     for (int iter = 0; iter < 3; ++iter) {
@@ -398,66 +404,105 @@ __global__ __launch_bounds__(64) void tri_eig_wave_kernel(const double* __restri
         double a = d[0] - lam;
         double c = n > 1 ? e[0] : 0.0;
         double yk = first ? lane_rand(0u, (uint32_t)b) : X[IDX(0)] * sc;
-        for (int k = 0; k + 1 < n; ++k) {
-            const double sub_ = e[k];
-            const double dn = d[k + 1] - lam;
-            const double en = (k + 2 < n) ? e[k + 1] : 0.0;
-            double yk1 = first ? lane_rand((uint32_t)(k + 1), (uint32_t)b) : X[IDX(k + 1)] * sc;
-            double mult;
-            if (fabs(a) >= fabs(sub_)) {
-                if (a == 0.0) a = tiny;
-                const double ra = rcp_nr64(a);
-                mult = sub_ * ra;
-                if (first) {
-                    U0[IDX(k)] = ra;
-                    U1[IDX(k)] = c;
-                    U2[IDX(k)] = 0.0;
-                }
-                a = dn - mult * c;
-                c = en;
-            } else {
-                const double rs = rcp_nr64(sub_);
-                mult = a * rs;
-                if (first) {
-                    U0[IDX(k)] = rs;
-                    U1[IDX(k)] = dn;
-                    U2[IDX(k)] = en;
-                }
-                a = c - mult * dn;
-                c = -mult * en;
-                const double tmp = yk;
-                yk = yk1;
-                yk1 = tmp;
+        for (int k0 = 0; k0 + 1 < n; k0 += PB) {
+            double xn[PB];
+            if (!first) {
+#pragma unroll
+                for (int u = 0; u < PB; ++u) xn[u] = (k0 + 1 + u < n) ? X[IDX(k0 + 1 + u)] : 0.0;
             }
-            yk1 -= mult * yk;
-            X[IDX(k)] = yk;
-            yk = yk1;
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int k = k0 + u;
+                if (k + 1 >= n) break;
+                const double sub_ = e[k];
+                const double dn = d[k + 1] - lam;
+                const double en = (k + 2 < n) ? e[k + 1] : 0.0;
+                double yk1 = first ? lane_rand((uint32_t)(k + 1), (uint32_t)b) : xn[u] * sc;
+                double mult;
+                if (fabs(a) >= fabs(sub_)) {
+                    if (a == 0.0) a = tiny;
+                    const double ra = rcp_nr64(a);
+                    mult = sub_ * ra;
+                    if (first) {
+                        U0[IDX(k)] = ra;
+                        U1[IDX(k)] = c;
+                        U2[IDX(k)] = 0.0;
+                    }
+                    a = dn - mult * c;
+                    c = en;
+                } else {
+                    const double rs = rcp_nr64(sub_);
+                    mult = a * rs;
+                    if (first) {
+                        U0[IDX(k)] = rs;
+                        U1[IDX(k)] = dn;
+                        U2[IDX(k)] = en;
+                    }
+                    a = c - mult * dn;
+                    c = -mult * en;
+                    const double tmp = yk;
+                    yk = yk1;
+                    yk1 = tmp;
+                }
+                yk1 -= mult * yk;
+                X[IDX(k)] = yk;
+                yk = yk1;
+            }
         }
-        if (first) {
-            if (a == 0.0) a = tiny;
-            U0[IDX(n - 1)] = rcp_nr64(a);
-        }
+        // the last pivot is the same every sweep
+        if (a == 0.0) a = tiny;
+        const double ra_last = rcp_nr64(a);
+        if (first) U0[IDX(n - 1)] = ra_last;
         // back substitution
-        double x2 = 0.0, x1 = yk * U0[IDX(n - 1)];
+        double x2 = 0.0, x1 = yk * ra_last;
         X[IDX(n - 1)] = x1;
         double amax = fabs(x1);
-        for (int k = n - 2; k >= 0; --k) {
-            const double xk = (X[IDX(k)] - U1[IDX(k)] * x1 - U2[IDX(k)] * x2) * U0[IDX(k)];
-            X[IDX(k)] = xk;
-            x2 = x1;
-            x1 = xk;
-            amax = fmax(amax, fabs(xk));
+        for (int k1 = n - 2; k1 >= 0; k1 -= PB) {
+            double xv[PB], u0[PB], u1[PB], u2[PB];
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int k = k1 - u;
+                if (k < 0) break;
+                xv[u] = X[IDX(k)];
+                u0[u] = U0[IDX(k)];
+                u1[u] = U1[IDX(k)];
+                u2[u] = U2[IDX(k)];
+            }
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int k = k1 - u;
+                if (k < 0) break;
+                const double xk = (xv[u] - u1[u] * x1 - u2[u] * x2) * u0[u];
+                X[IDX(k)] = xk;
+                x2 = x1;
+                x1 = xk;
+                amax = fmax(amax, fabs(xk));
+            }
         }
         sc = 1.0 / amax;
     }
     double nrm = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double xi = X[IDX(i)] * sc;
-        nrm += xi * xi;
+    for (int i0 = 0; i0 < n; i0 += PB) {
+        double xv[PB];
+#pragma unroll
+        for (int u = 0; u < PB; ++u) xv[u] = (i0 + u < n) ? X[IDX(i0 + u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            if (i0 + u >= n) break;
+            const double xi = xv[u] * sc;
+            nrm += xi * xi;
+        }
     }
     const double sn = 1.0 / sqrt(nrm);
     double* z = Z + (size_t)t * n;
-    for (int i = 0; i < n; ++i) z[i] = (X[IDX(i)] * sc) * sn;
+    for (int i0 = 0; i0 < n; i0 += PB) {
+        double xv[PB];
+#pragma unroll
+        for (int u = 0; u < PB; ++u) xv[u] = (i0 + u < n) ? X[IDX(i0 + u)] : 0.0;
+#pragma unroll
+        for (int u = 0; u < PB; ++u)
+            if (i0 + u < n) z[i0 + u] = (xv[u] * sc) * sn;
+    }
 #undef IDX
 }
 
@@ -579,9 +624,10 @@ constexpr int kBtBS = 256;
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s) {
-    // one KState per basis vector in LDS: up to 890 vectors in the 160 KB of a CU
+    // one KState per basis vector in LDS beside the phase table: up to 860
+    // vectors in the 160 KB of a CU
     const size_t shm = sizeof(KState) * (size_t)P.N;
-    if (shm > 163840) return hipErrorInvalidValue;
+    if (shm + sizeof(double2) * kVPhaseTab > 163840) return hipErrorInvalidValue;
     auto kern = assemble_kernel<kAsmBS>;
     hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (err != hipSuccess) return err;

@@ -374,6 +374,7 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     P.need_l0_c = (m->R0_cation != 0.0) && (m->alpha_0_cation != 0.0 || m->beta_0_cation != 0.0);
     P.need_l2_a = m->A2_anion != 0.0;
     P.need_l2_c = m->A2_cation != 0.0;
+    P.same_r0 = P.need_l0_a && P.need_l0_c && m->R0_anion == m->R0_cation;
     // SpinOrbitFunctional.cpp
     P.soc_kfac = (2.0 * kPi) / a;
     P.period_a = m->period_anion;
