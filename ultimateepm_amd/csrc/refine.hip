@@ -384,11 +384,15 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
                                                          const double* __restrict__ kpts,
                                                          const int32_t* __restrict__ G) {
     constexpr int NW = BS / 64;
-    __shared__ double part[NW][4][16][16];  // per-wave partial S_re, S_im, M_re, M_im
-    __shared__ double2 Mm[16 * 16];
-    __shared__ double2 Ss[16 * 16];
-    __shared__ double2 Zm[16 * 16];
-    __shared__ double2 Wt[16 * 16];
+    // per-wave partial S_re, S_im, M_re, M_im; once S and M are summed (in
+    // registers) the same LDS holds the small problem's M, S, Z and W, so a
+    // workgroup takes 33 KB and four fit on a CU
+    __shared__ __align__(16) double part[NW][4][16][16];
+    static_assert(sizeof(part) >= 4 * 16 * 16 * sizeof(double2), "M, S, Z and W fit in the partial slots");
+    double2* Mm = reinterpret_cast<double2*>(&part[0][0][0][0]);
+    double2* Ss = Mm + 16 * 16;
+    double2* Zm = Ss + 16 * 16;
+    double2* Wt = Zm + 16 * 16;
     __shared__ double th[16];
     __shared__ int perm[16];
     __shared__ double4 rot[9];
@@ -456,8 +460,9 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
         part[wv][3][r][l16] = mim[v];
     }
     __syncthreads();
-    for (int e = tid; e < 256; e += BS) {
-        const int a = e >> 4, b = e & 15;
+    static_assert(BS == 256, "one (a, b) entry of S and M per thread");
+    {
+        const int a = tid >> 4, b = tid & 15;
         double s0 = 0, s1 = 0, m0 = 0, m1 = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
@@ -466,6 +471,7 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
             m0 += part[w][2][a][b];
             m1 += part[w][3][a][b];
         }
+        __syncthreads();  // every partial is read before Ss and Mm overwrite them
         if (a < nbr && b < nbr) {
             Ss[a * 16 + b] = make_double2(s0, s1);
             Mm[a * 16 + b] = make_double2(m0, m1);
