@@ -62,52 +62,116 @@ constexpr int kUnroll2 = 4;
 // Cholesky S = L L^H, M' = L^{-1} M L^{-H}, parallel-ordered Jacobi M' = Z Theta Z^H,
 // C = L^{-H} Z (in Wt), Ritz values th and the ascending order perm.
 // Inputs Ss = S, Mm = M (row-major, stride LDM); Ss, Mm, Zm, Wt are overwritten.
+// Triangular solves on one column held in registers (v[r], r < nbr), with L
+// lower triangular in Ss (row-major, stride LDM, real diagonal); each row
+// subtracts its terms in ascending k, as a column-by-column substitution does.
 template <int LDM>
-__device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* Zm, double2* Wt, double* th,
-                                    int* perm, double4* rot, int lane) {
+__device__ __forceinline__ void lower_solve(int nbr, const double2* __restrict__ L, double2 (&v)[LDM]) {
+#pragma unroll
+    for (int r = 0; r < LDM; ++r)
+        if (r < nbr) {
+            double2 s = v[r];
+#pragma unroll
+            for (int k = 0; k < r; ++k) s = zsub(s, zmul(L[r * LDM + k], v[k]));
+            v[r] = zscale(s, 1.0 / L[r * LDM + r].x);
+        }
+}
+// L^H v' = v (back substitution)
+template <int LDM>
+__device__ __forceinline__ void upper_solve_h(int nbr, const double2* __restrict__ L, double2 (&v)[LDM]) {
+#pragma unroll
+    for (int r = LDM - 1; r >= 0; --r)
+        if (r < nbr) {
+            double2 s = v[r];
+#pragma unroll
+            for (int k = r + 1; k < LDM; ++k)
+                if (k < nbr) s = zsub(s, zmulc(L[k * LDM + r], v[k]));  // (L^H)[r][k] = conj(L[k][r])
+            v[r] = zscale(s, 1.0 / L[r * LDM + r].x);
+        }
+}
+
+// The small Hermitian problem on nbr <= LDM Ritz vectors, in one wave:
+// Cholesky S = L L^H, M' = L^{-1} M L^{-H}, parallel-ordered Jacobi M' = Z Theta Z^H,
+// C = L^{-H} Z (in Wt), Ritz values th and the ascending order perm.
+// Inputs Ss = S, Mm = M (row-major, stride LDM); Ss, Mm, Zm, Wt are overwritten.
+template <int LDM>
+__device__ void small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* __restrict__ Mm,
+                                    double2* __restrict__ Zm, double2* __restrict__ Wt, double* __restrict__ th,
+                                    int* __restrict__ perm, double4* __restrict__ rot, int lane) {
     const int i = lane;
+    // Cholesky, right-looking: column j is scaled, then the trailing lower
+    // triangle takes its rank-1 update, the elements spread over the lanes
+    // (each element subtracts its terms in ascending j, as a left-looking
+    // factorisation does)
     for (int j = 0; j < nbr; ++j) {
-        double djj = Ss[j * LDM + j].x;
-        for (int k = 0; k < j; ++k) {
-            const double2 l = Ss[j * LDM + k];
-            djj -= l.x * l.x + l.y * l.y;
-        }
-        const double ljj = sqrt(djj);
-        if (i > j && i < nbr) {
-            // L[i][j] = (S[i][j] - sum_k L[i][k] conj(L[j][k])) / L[j][j]
-            double2 s = Ss[i * LDM + j];
-            for (int k = 0; k < j; ++k)
-                s = zsub(s, zmul(Ss[i * LDM + k], make_double2(Ss[j * LDM + k].x, -Ss[j * LDM + k].y)));
-            Ss[i * LDM + j] = zscale(s, 1.0 / ljj);
-        }
+        const double ljj = sqrt(Ss[j * LDM + j].x);
+        const double rl = 1.0 / ljj;
+        wsync();
+        if (i > j && i < nbr) Ss[i * LDM + j] = zscale(Ss[i * LDM + j], rl);
         if (i == 0) Ss[j * LDM + j] = make_double2(ljj, 0.0);
         wsync();
-    }
-    // Wt = L^{-1} M  (lane = column of M)
-    if (i < nbr) {
-        for (int r = 0; r < nbr; ++r) {
-            double2 s = Mm[r * LDM + i];
-            for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Wt[k * LDM + i]));
-            Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
+        for (int e = i; e < LDM * LDM; e += 64) {
+            const int r = e / LDM, c = e % LDM;
+            if (c > j && r >= c && r < nbr) {
+                const double2 lc = Ss[c * LDM + j];
+                Ss[r * LDM + c] = zsub(Ss[r * LDM + c], zmul(Ss[r * LDM + j], make_double2(lc.x, -lc.y)));
+            }
         }
+        wsync();
     }
-    wsync();
-    // M' = (L^{-1} Wt^H)^H  (lane = column of Wt^H, i.e. row i of Wt)
-    if (i < nbr) {
-        for (int r = 0; r < nbr; ++r) {
-            const double2 w = Wt[i * LDM + r];
-            double2 s = make_double2(w.x, -w.y);
-            for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Zm[k * LDM + i]));
-            Zm[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);  // (L^{-1} Wt^H)[r][i]
+    // Wt = L^{-1} M  (lane = column of M), then M' = (L^{-1} Wt^H)^H (lane =
+    // column of Wt^H, i.e. row i of Wt): with U = L^{-1} Wt^H, M'[i][c] =
+    // conj(U[c][i]).  Up to 16 bands the columns stay in registers; wider
+    // problems substitute through LDS (the same operations in the same order).
+    if constexpr (LDM <= 16) {
+        if (i < nbr) {
+            double2 v[LDM];
+#pragma unroll
+            for (int r = 0; r < LDM; ++r) v[r] = r < nbr ? Mm[r * LDM + i] : make_double2(0.0, 0.0);
+            lower_solve<LDM>(nbr, Ss, v);
+#pragma unroll
+            for (int r = 0; r < LDM; ++r)
+                if (r < nbr) Wt[r * LDM + i] = v[r];
         }
+        wsync();
+        if (i < nbr) {
+            double2 v[LDM];
+#pragma unroll
+            for (int r = 0; r < LDM; ++r) {
+                const double2 w = r < nbr ? Wt[i * LDM + r] : make_double2(0.0, 0.0);
+                v[r] = make_double2(w.x, -w.y);
+            }
+            lower_solve<LDM>(nbr, Ss, v);
+#pragma unroll
+            for (int c = 0; c < LDM; ++c)
+                if (c < nbr) Mm[i * LDM + c] = make_double2(v[c].x, -v[c].y);
+        }
+        wsync();
+    } else {
+        if (i < nbr) {
+            for (int r = 0; r < nbr; ++r) {
+                double2 s = Mm[r * LDM + i];
+                for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Wt[k * LDM + i]));
+                Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
+            }
+        }
+        wsync();
+        if (i < nbr) {
+            for (int r = 0; r < nbr; ++r) {
+                const double2 w = Wt[i * LDM + r];
+                double2 s = make_double2(w.x, -w.y);
+                for (int k = 0; k < r; ++k) s = zsub(s, zmul(Ss[r * LDM + k], Zm[k * LDM + i]));
+                Zm[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);  // U[r][i]
+            }
+        }
+        wsync();
+        if (i < nbr)
+            for (int c = 0; c < nbr; ++c) {
+                const double2 u = Zm[c * LDM + i];
+                Mm[i * LDM + c] = make_double2(u.x, -u.y);
+            }
+        wsync();
     }
-    wsync();
-    if (i < nbr)
-        for (int c = 0; c < nbr; ++c) {
-            const double2 u = Zm[c * LDM + i];  // M'[i][c] = conj(U[c][i])
-            Mm[i * LDM + c] = make_double2(u.x, -u.y);
-        }
-    wsync();
     if (i < nbr)
         for (int c = 0; c < nbr; ++c) Zm[i * LDM + c] = make_double2(i == c ? 1.0 : 0.0, 0.0);
     if (i < nbr) {
@@ -125,10 +189,33 @@ __device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* 
     // is rotated while its off-diagonal element still moves an eigenvalue by more
     // than ~1e-13 of the local scale (|c|^2 / gap) or is not negligible outright.
     const int m = (nbr + 1) & ~1, half = m >> 1;
+    // the rotation test of a pair (p < q)
+    auto negligible = [&](int p, int q) {
+        const double2 c = Mm[p * LDM + q];
+        const double ac = sqrt(c.x * c.x + c.y * c.y);
+        const double a = Mm[p * LDM + p].x, b = Mm[q * LDM + q].x;
+        const double scale = fabs(a) + fabs(b) + 1e-300;
+        return ac <= 1e-15 * scale || ac * ac <= 1e-13 * scale * fabs(b - a);
+    };
     for (int sweep = 0; sweep < 20; ++sweep) {
+        // a sweep that finds every pair negligible at its start rotates nothing
+        // (no round changes M): test all pairs at once and stop
+        {
+            int big = 0;
+            for (int pq = i; pq < LDM * LDM; pq += 64) {
+                const int p = pq / LDM, q = pq % LDM;
+                if (p < q && q < nbr && !negligible(p, q)) big = 1;
+            }
+            for (int o = 32; o > 0; o >>= 1) big |= __shfl_xor(big, o, 64);
+            if (!big) break;
+        }
         int rotated = 0;
         for (int round = 0; round < m - 1; ++round) {
-            auto elem = [&](int pos) { return pos == 0 ? 0 : ((pos - 1 + round) % (m - 1)) + 1; };
+            auto elem = [&](int pos) {
+                if (pos == 0) return 0;
+                const int v = pos - 1 + round;  // < 2 (m - 1)
+                return (v >= m - 1 ? v - (m - 1) : v) + 1;
+            };
             // rotation parameters, one lane per pair
             int did = 0;
             if (i < half) {
@@ -139,9 +226,7 @@ __device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* 
                     const double2 c = Mm[p * LDM + q];
                     const double ac = sqrt(c.x * c.x + c.y * c.y);
                     const double a = Mm[p * LDM + p].x, b = Mm[q * LDM + q].x;
-                    const double scale = fabs(a) + fabs(b) + 1e-300;
-                    const bool tiny = ac <= 1e-15 * scale || ac * ac <= 1e-13 * scale * fabs(b - a);
-                    if (!tiny) {
+                    if (!negligible(p, q)) {
                         did = 1;
                         const double emx = c.x / ac, emy = -c.y / ac;  // e^{-i phi}, c = |c| e^{i phi}
                         const double tau_ = (b - a) / (2.0 * ac);
@@ -157,8 +242,9 @@ __device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* 
             if (did) {
                 rotated = 1;
                 // columns: M <- M G, Z <- Z G with G = [[cs, sn], [-sn e, cs e]] on (p, q)
-                for (int task = i; task < half * nbr; task += 64) {
-                    const int k = task / nbr, row = task % nbr;
+                for (int task = i; task < half * LDM; task += 64) {
+                    const int k = task / LDM, row = task % LDM;
+                    if (row >= nbr) continue;
                     int p = elem(k), q = elem(m - 1 - k);
                     if (p > q) { const int t = p; p = q; q = t; }
                     if (q >= nbr) continue;
@@ -175,8 +261,9 @@ __device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* 
                 }
                 wsync();
                 // rows: M <- G^H M
-                for (int task = i; task < half * nbr; task += 64) {
-                    const int k = task / nbr, col = task % nbr;
+                for (int task = i; task < half * LDM; task += 64) {
+                    const int k = task / LDM, col = task % LDM;
+                    if (col >= nbr) continue;
                     int p = elem(k), q = elem(m - 1 - k);
                     if (p > q) { const int t = p; p = q; q = t; }
                     if (q >= nbr) continue;
@@ -205,76 +292,117 @@ __device__ void small_rayleigh_ritz(int nbr, double2* Ss, double2* Mm, double2* 
     }
     wsync();
     // C = L^{-H} Z (back substitution with L^H, lane = column of Z)
-    if (i < nbr) {
-        for (int r = nbr - 1; r >= 0; --r) {
-            double2 s = Zm[r * LDM + i];
-            for (int k = r + 1; k < nbr; ++k) {
-                const double2 l = Ss[k * LDM + r];  // (L^H)[r][k] = conj(L[k][r])
-                s = zsub(s, zmulc(l, Wt[k * LDM + i]));
+    if constexpr (LDM <= 16) {
+        if (i < nbr) {
+            double2 v[LDM];
+#pragma unroll
+            for (int r = 0; r < LDM; ++r) v[r] = r < nbr ? Zm[r * LDM + i] : make_double2(0.0, 0.0);
+            upper_solve_h<LDM>(nbr, Ss, v);
+#pragma unroll
+            for (int r = 0; r < LDM; ++r)
+                if (r < nbr) Wt[r * LDM + i] = v[r];
+        }
+    } else {
+        if (i < nbr) {
+            for (int r = nbr - 1; r >= 0; --r) {
+                double2 s = Zm[r * LDM + i];
+                for (int k = r + 1; k < nbr; ++k) s = zsub(s, zmulc(Ss[k * LDM + r], Wt[k * LDM + i]));
+                Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
             }
-            Wt[r * LDM + i] = zscale(s, 1.0 / Ss[r * LDM + r].x);
         }
     }
-    if (i < nbr) th[i] = Mm[i * LDM + i].x;
+    if (i < nbr) {
+        th[i] = Mm[i * LDM + i].x;
+        perm[i] = i;  // keeps perm a valid index map should a rank below collide (NaN)
+    }
     wsync();
-    if (i == 0) {
-        for (int b = 0; b < nbr; ++b) perm[b] = b;
-        for (int b = 0; b < nbr; ++b) {  // selection sort of the Ritz values
-            int best = b;
-            for (int c = b + 1; c < nbr; ++c)
-                if (th[perm[c]] < th[perm[best]]) best = c;
-            const int tmp = perm[b];
-            perm[b] = perm[best];
-            perm[best] = tmp;
+    // ascending order: the rank of Ritz value i (ties by index)
+    if (i < nbr) {
+        const double ti = th[i];
+        int rank = 0;
+        for (int c = 0; c < nbr; ++c) {
+            const double tc = th[c];
+            rank += (tc < ti || (tc == ti && c < i)) ? 1 : 0;
         }
+        perm[rank] = i;
     }
     wsync();
 }
 
 // X1 = X0 C[:, perm[b]] for the lowest nb Ritz pairs: eigenvalues, unit
-// eigenvectors and Hellmann-Feynman gradients (Hamiltonian.cpp:146-194).
+// eigenvectors and Hellmann-Feynman gradients (Hamiltonian.cpp:146-194).  One
+// thread per row and up to eight bands per pass, so a pass reads X0 once;
+// red: LDS scratch of 24 (BS / 64) doubles that is free at this point.
 template <int BS, int LDM>
-__device__ void write_ritz(const double2* X, int n, int nbr, int nb, const double2* Wt, const double* th,
-                           const int* perm, int64_t kk, double* evals, double2* evecs, double* grads,
-                           const DevParams& P, const double* kpts, const int32_t* G) {
+__device__ void write_ritz(const double2* __restrict__ X, int n, int nbr, int nb, const double2* Wt,
+                           const double* th, const int* perm, int64_t kk, double* evals, double2* evecs,
+                           double* grads, const DevParams& P, const double* kpts, const int32_t* G, double* red) {
+    constexpr int NW = BS / 64, BC = 8;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid < nb) evals[kk * nb + tid] = th[perm[tid]];
+    if (!evecs && !grads) return;
     double2* out = evecs ? evecs + kk * (int64_t)nb * n : nullptr;
-    for (int b = wv; b < nb; b += BS / 64) {
-        const int cb = perm[b];
-        double gx = 0, gy = 0, gz = 0;
-        const double* k = kpts + 3 * kk;
-        for (int r = lane; r < n; r += 64) {
-            double2 acc = make_double2(0.0, 0.0);
+    const double* k = kpts + 3 * kk;
+    for (int b0 = 0; b0 < nb; b0 += BC) {
+        double gx[BC], gy[BC], gz[BC];
+#pragma unroll
+        for (int u = 0; u < BC; ++u) gx[u] = gy[u] = gz[u] = 0.0;
+        for (int r = tid; r < n; r += BS) {
+            double2 acc[BC];
+#pragma unroll
+            for (int u = 0; u < BC; ++u) acc[u] = make_double2(0.0, 0.0);
             for (int a = 0; a < nbr; ++a) {
                 const double2 x = X[a * n + r];
-                const double2 c = Wt[a * LDM + cb];
-                acc.x = fma(x.x, c.x, fma(-x.y, c.y, acc.x));
-                acc.y = fma(x.x, c.y, fma(x.y, c.x, acc.y));
+#pragma unroll
+                for (int u = 0; u < BC; ++u)
+                    if (b0 + u < nb) {
+                        const double2 c = Wt[a * LDM + perm[b0 + u]];
+                        acc[u].x = fma(x.x, c.x, fma(-x.y, c.y, acc[u].x));
+                        acc[u].y = fma(x.x, c.y, fma(x.y, c.x, acc[u].y));
+                    }
             }
-            if (out) out[(int64_t)b * n + r] = acc;
-            if (grads) {
-                // Hellmann-Feynman, kinetic term (Hamiltonian.cpp:146-194); SOC: spin-down row r
-                // of basis vector r - N adds its weight to the same k + G.
-                const int i = r < P.N ? r : r - P.N;
-                const double wt = acc.x * acc.x + acc.y * acc.y;
-                gx += (k[0] + (double)G[3 * i]) * wt;
-                gy += (k[1] + (double)G[3 * i + 1]) * wt;
-                gz += (k[2] + (double)G[3 * i + 2]) * wt;
-            }
+#pragma unroll
+            for (int u = 0; u < BC; ++u)
+                if (b0 + u < nb) {
+                    if (out) out[(int64_t)(b0 + u) * n + r] = acc[u];
+                    if (grads) {
+                        // Hellmann-Feynman, kinetic term (Hamiltonian.cpp:146-194); SOC: spin-down
+                        // row r of basis vector r - N adds its weight to the same k + G.
+                        const int i = r < P.N ? r : r - P.N;
+                        const double wt = acc[u].x * acc[u].x + acc[u].y * acc[u].y;
+                        gx[u] += (k[0] + (double)G[3 * i]) * wt;
+                        gy[u] += (k[1] + (double)G[3 * i + 1]) * wt;
+                        gz[u] += (k[2] + (double)G[3 * i + 2]) * wt;
+                    }
+                }
         }
         if (grads) {
-            for (int o = 32; o > 0; o >>= 1) {
-                gx += __shfl_xor(gx, o, 64);
-                gy += __shfl_xor(gy, o, 64);
-                gz += __shfl_xor(gz, o, 64);
+            // wave sums, then the waves' partials in LDS, added in wave order
+#pragma unroll
+            for (int u = 0; u < BC; ++u)
+                for (int o = 32; o > 0; o >>= 1) {
+                    gx[u] += __shfl_xor(gx[u], o, 64);
+                    gy[u] += __shfl_xor(gy[u], o, 64);
+                    gz[u] += __shfl_xor(gz[u], o, 64);
+                }
+            if (lane == 0)
+#pragma unroll
+                for (int u = 0; u < BC; ++u) {
+                    red[(wv * BC + u) * 3 + 0] = gx[u];
+                    red[(wv * BC + u) * 3 + 1] = gy[u];
+                    red[(wv * BC + u) * 3 + 2] = gz[u];
+                }
+            __syncthreads();
+            if (tid < 3 * BC) {
+                const int u = tid / 3, c = tid % 3;
+                if (b0 + u < nb) {
+                    double sum = 0.0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) sum += red[(w * BC + u) * 3 + c];
+                    grads[3 * (kk * nb + b0 + u) + c] = sum * P.grad_pref * P.frac_to_phys;
+                }
             }
-            if (lane == 0) {
-                double* g = grads + 3 * (kk * nb + b);
-                g[0] = gx * P.grad_pref * P.frac_to_phys;
-                g[1] = gy * P.grad_pref * P.frac_to_phys;
-                g[2] = gz * P.grad_pref * P.frac_to_phys;
-            }
+            __syncthreads();
         }
     }
 }
@@ -361,7 +489,8 @@ __global__ __launch_bounds__(BS) void refine_kernel(const double2* __restrict__ 
     // ---- small dense problem in wave 0
     if (wv == 0) small_rayleigh_ritz<kNbrMax>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
     __syncthreads();
-    write_ritz<BS, kNbrMax>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
+    write_ritz<BS, kNbrMax>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
+                            reinterpret_cast<double*>(Mm));
 }
 
 // ---------------------------------------------------------------------------
@@ -377,7 +506,7 @@ __device__ __forceinline__ dbl4 mfma64(double a, double b, dbl4 c) {
 }
 
 template <int BS>
-__global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restrict__ H, int n, int ld,
+__global__ __launch_bounds__(BS, 4) void refine_mfma_kernel(const double2* __restrict__ H, int n, int ld,
                                                          const double2* __restrict__ Xall, int nbr, int nb,
                                                          double* __restrict__ evals, double2* __restrict__ evecs,
                                                          double* __restrict__ grads, DevParams P,
@@ -480,7 +609,8 @@ __global__ __launch_bounds__(BS) void refine_mfma_kernel(const double2* __restri
     __syncthreads();
     if (wv == 0) small_rayleigh_ritz<16>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
     __syncthreads();
-    write_ritz<BS, 16>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
+    write_ritz<BS, 16>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
+                       reinterpret_cast<double*>(Mm));
 }
 
 // 16 < nbr <= 32: the same scheme on two 16-band tiles.  A lane's MFMA column
@@ -614,7 +744,8 @@ __global__ __launch_bounds__(BS, 2) void refine_mfma2_kernel(const double2* __re
     __syncthreads();  // part is free from here: Zm and Wt take its place
     if (wv == 0) small_rayleigh_ritz<32>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
     __syncthreads();
-    write_ritz<BS, 32>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G);
+    write_ritz<BS, 32>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
+                       reinterpret_cast<double*>(Mm));
 }
 
 const char* refine_kernel_name(int nbr) {
