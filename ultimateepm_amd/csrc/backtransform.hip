@@ -15,7 +15,9 @@
 //                        ahead of their use.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 
 #include "epm_physics.hpp"
 #include "solver.hpp"
@@ -113,34 +115,38 @@ __global__ __launch_bounds__(64) void cluster_orth_kernel(const double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// x = Q z for BPW * (blockDim / 64) bands per workgroup; R is the arithmetic type,
-// VT the storage of the reflectors (column j of V holds v_j at rows j+2..n-1;
-// v_j[j+1] = 1).
-template <typename R, typename VT, int ROWS>
+// x = Q z for BPL bands per 16-lane DPP row (blockDim / 16 rows per workgroup;
+// band (blockIdx.y * rows + row) * BPL + u); R is the arithmetic type, VT the
+// storage of the reflectors (column j of V holds v_j at rows j+2..n-1;
+// v_j[j+1] = 1).  A reflector row loaded by a lane serves its BPL bands.
+template <typename R, typename VT, int ROWS, int BPL>
 __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, int n, int ld,
                                                       const VT* __restrict__ tauall, const double* __restrict__ Z,
                                                       int nb, double2* __restrict__ X, double* __restrict__ grads,
                                                       DevParams P, const double* __restrict__ kpts,
                                                       const int32_t* __restrict__ G) {
-    // register budget: prefetch the next reflector when two copies of v fit,
-    // keep one copy when it fits, else re-read v (an L1 hit) for the update
+    // register budget: prefetch the next reflector when two copies of v fit
+    // beside the bands, keep one copy when it fits, else re-read v (an L1 hit)
     constexpr int CW = 2 * (int)sizeof(R) / 4;  // VGPRs per complex
-    constexpr bool KEEP = 2 * ROWS * CW <= 200;
-    constexpr bool PREFETCH = 3 * ROWS * CW <= 200;
+    constexpr bool KEEP = (BPL + 1) * ROWS * CW <= 200;
+    constexpr bool PREFETCH = (BPL + 2) * ROWS * CW <= 200;
     const int64_t kk = blockIdx.x;
     const int tid = threadIdx.x, rs = tid & 15;
-    const int b = blockIdx.y * (blockDim.x >> 4) + (tid >> 4);
-    const bool live = b < nb;
+    const int b0 = (blockIdx.y * (blockDim.x >> 4) + (tid >> 4)) * BPL;
     const VT* Vk = V + kk * (int64_t)ld * n;
     const VT* tau = tauall + kk * n;
-    const double* z = Z + (kk * nb + (live ? b : 0)) * (int64_t)n;
 
-    C2<R> x[ROWS];
+    C2<R> x[BPL][ROWS];
 #pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-        const int i = rs + 16 * q;
-        x[q].x = (live && i < n) ? (R)z[i] : (R)0;
-        x[q].y = (R)0;
+    for (int u = 0; u < BPL; ++u) {
+        const bool live = b0 + u < nb;
+        const double* z = Z + (kk * nb + (live ? b0 + u : 0)) * (int64_t)n;
+#pragma unroll
+        for (int q = 0; q < ROWS; ++q) {
+            const int i = rs + 16 * q;
+            x[u][q].x = (live && i < n) ? (R)z[i] : (R)0;
+            x[u][q].y = (R)0;
+        }
     }
     // v_j at this lane's row i; zero at rows <= j+1 and >= n (v_j[j+1] = 1 is set by the caller)
     auto vrow = [&](int j, int q) -> C2<R> {
@@ -167,28 +173,40 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
         if (!(t.x == (R)0 && t.y == (R)0)) {  // wave-uniform
             const int q0 = (j + 1) >> 4;  // first block of rows that can be nonzero
             const int r1 = (j + 1) & 15;  // lane holding row j+1 in block q0
-            R sr = 0, si = 0;
+            R sr[BPL], si[BPL];
+#pragma unroll
+            for (int u = 0; u < BPL; ++u) sr[u] = si[u] = (R)0;
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
                 if (q >= q0) {
                     C2<R> vq = KEEP ? v[KEEP ? q : 0] : vrow(j, q);
                     if (q == q0 && rs == r1) vq = {(R)1, (R)0};
                     // conj(v) x
-                    sr = fma(vq.x, x[q].x, fma(vq.y, x[q].y, sr));
-                    si = fma(vq.x, x[q].y, fma(-vq.y, x[q].x, si));
+#pragma unroll
+                    for (int u = 0; u < BPL; ++u) {
+                        sr[u] = fma(vq.x, x[u][q].x, fma(vq.y, x[u][q].y, sr[u]));
+                        si[u] = fma(vq.x, x[u][q].y, fma(-vq.y, x[u][q].x, si[u]));
+                    }
                 }
             }
-            sr = row_sum16(sr);
-            si = row_sum16(si);
-            // s = tau (v^H x); x -= s v
-            const R pr = t.x * sr - t.y * si, pi = t.x * si + t.y * sr;
+            R pr[BPL], pi[BPL];
+#pragma unroll
+            for (int u = 0; u < BPL; ++u) {
+                const R ar = row_sum16(sr[u]), ai = row_sum16(si[u]);
+                // s = tau (v^H x); x -= s v
+                pr[u] = t.x * ar - t.y * ai;
+                pi[u] = t.x * ai + t.y * ar;
+            }
 #pragma unroll
             for (int q = 0; q < ROWS; ++q) {
                 if (q >= q0) {
                     C2<R> vq = KEEP ? v[KEEP ? q : 0] : vrow(j, q);
                     if (q == q0 && rs == r1) vq = {(R)1, (R)0};
-                    x[q].x = fma(-pr, vq.x, fma(pi, vq.y, x[q].x));
-                    x[q].y = fma(-pr, vq.y, fma(-pi, vq.x, x[q].y));
+#pragma unroll
+                    for (int u = 0; u < BPL; ++u) {
+                        x[u][q].x = fma(-pr[u], vq.x, fma(pi[u], vq.y, x[u][q].x));
+                        x[u][q].y = fma(-pr[u], vq.y, fma(-pi[u], vq.x, x[u][q].y));
+                    }
                 }
             }
         }
@@ -197,61 +215,92 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
             for (int q = 0; q < ROWS; ++q) v[KEEP ? q : 0] = vn[PREFETCH ? q : 0];
         }
     }
-    if (!live) return;
-    double2* xo = X + (kk * nb + b) * (int64_t)n;
 #pragma unroll
-    for (int q = 0; q < ROWS; ++q) {
-        const int i = rs + 16 * q;
-        if (i < n) xo[i] = make_double2((double)x[q].x, (double)x[q].y);
-    }
-    if (grads) {
-        // Hellmann-Feynman on the kinetic term; spinor rows i >= N belong to G_{i-N}
-        const double* k = kpts + 3 * kk;
-        const int N = P.N;
-        double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int u = 0; u < BPL; ++u) {
+        const int b = b0 + u;
+        if (b >= nb) break;
+        double2* xo = X + (kk * nb + b) * (int64_t)n;
 #pragma unroll
         for (int q = 0; q < ROWS; ++q) {
             const int i = rs + 16 * q;
-            if (i < n) {
-                const int g = i < N ? i : i - N;
-                const double w = (double)x[q].x * (double)x[q].x + (double)x[q].y * (double)x[q].y;
-                sx += (k[0] + (double)G[3 * g]) * w;
-                sy += (k[1] + (double)G[3 * g + 1]) * w;
-                sz += (k[2] + (double)G[3 * g + 2]) * w;
-            }
+            if (i < n) xo[i] = make_double2((double)x[u][q].x, (double)x[u][q].y);
         }
-        sx = row_sum16(sx);
-        sy = row_sum16(sy);
-        sz = row_sum16(sz);
-        if (rs == 0) {
-            double* gg = grads + 3 * (kk * nb + b);
-            gg[0] = sx * P.grad_pref * P.frac_to_phys;
-            gg[1] = sy * P.grad_pref * P.frac_to_phys;
-            gg[2] = sz * P.grad_pref * P.frac_to_phys;
+        if (grads) {
+            // Hellmann-Feynman on the kinetic term; spinor rows i >= N belong to G_{i-N}
+            const double* k = kpts + 3 * kk;
+            const int N = P.N;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+#pragma unroll
+            for (int q = 0; q < ROWS; ++q) {
+                const int i = rs + 16 * q;
+                if (i < n) {
+                    const int g = i < N ? i : i - N;
+                    const double w = (double)x[u][q].x * (double)x[u][q].x + (double)x[u][q].y * (double)x[u][q].y;
+                    sx += (k[0] + (double)G[3 * g]) * w;
+                    sy += (k[1] + (double)G[3 * g + 1]) * w;
+                    sz += (k[2] + (double)G[3 * g + 2]) * w;
+                }
+            }
+            sx = row_sum16(sx);
+            sy = row_sum16(sy);
+            sz = row_sum16(sz);
+            if (rs == 0) {
+                double* gg = grads + 3 * (kk * nb + b);
+                gg[0] = sx * P.grad_pref * P.frac_to_phys;
+                gg[1] = sy * P.grad_pref * P.frac_to_phys;
+                gg[2] = sz * P.grad_pref * P.frac_to_phys;
+            }
         }
     }
 }
 
 // ------------------------------------------------------------------ launchers
 namespace {
+// Bands per 16-lane row of apply_q_kernel (1, 2 or 4): as many as keep the
+// kernel within its registers at two or more waves per SIMD -- complex64 4 up
+// to 192 rows and 2 up to 256, complex128 2 up to 192 -- or UEPM_BT_BPL.
+int apply_bands_per_lane(bool f32, int n) {
+    static const int forced = [] {
+        const char* e = std::getenv("UEPM_BT_BPL");
+        const int b = e ? std::atoi(e) : 0;
+        return b >= 4 ? 4 : b == 2 ? 2 : b == 1 ? 1 : 0;
+    }();
+    if (forced) return forced;
+    if (f32) return n <= 192 ? 4 : n <= 256 ? 2 : 1;
+    return n <= 192 ? 2 : 1;
+}
+
 template <typename R, typename VT>
 hipError_t launch_apply(const VT* dV, int n, int ld, const VT* dtau, const double* dZ, int nb, double2* dX,
                         double* dgrads, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
                         int bands_per_group, hipStream_t s) {
-    // 16 lanes per band, bands_per_group (4, 8 or 16) bands per workgroup
+    // bands_per_group (4, 8 or 16) bands per workgroup, BPL of them per 16-lane row
     const int cap = (bands_per_group == 4 || bands_per_group == 8) ? bands_per_group : 16;
     const int bpw = nb >= cap ? cap : ((nb + 3) / 4) * 4;
-    const dim3 grid((unsigned)nk, (unsigned)((nb + bpw - 1) / bpw)), block(16 * bpw);
-#define UEPM_APPLY(ROWS)                                                                                       \
-    hipLaunchKernelGGL((apply_q_kernel<R, VT, ROWS>), grid, block, 0, s, dV, n, ld, dtau, dZ, nb, dX, dgrads, P, \
-                       dk, dG)
-    if (n <= 64) UEPM_APPLY(4);
-    else if (n <= 128) UEPM_APPLY(8);
-    else if (n <= 192) UEPM_APPLY(12);
-    else if (n <= 256) UEPM_APPLY(16);
-    else if (n <= 384) UEPM_APPLY(24);
-    else if (n <= 576) UEPM_APPLY(36);
+    // 1, 2 or 4 bands per row (a divisor of bpw, a multiple of 4) and at
+    // least four rows, one full wave, per workgroup
+    int bpl = apply_bands_per_lane(sizeof(R) == sizeof(float), n);
+    while (bpl > 1 && bpw / bpl < 4) bpl >>= 1;
+    const dim3 grid((unsigned)nk, (unsigned)((nb + bpw - 1) / bpw)), block(16 * (bpw / bpl));
+#define UEPM_APPLY(ROWS, BPL)                                                                                       \
+    hipLaunchKernelGGL((apply_q_kernel<R, VT, ROWS, BPL>), grid, block, 0, s, dV, n, ld, dtau, dZ, nb, dX, dgrads, \
+                       P, dk, dG)
+#define UEPM_APPLY_ROWS(BPL)                    \
+    if (n <= 64) UEPM_APPLY(4, BPL);            \
+    else if (n <= 128) UEPM_APPLY(8, BPL);      \
+    else if (n <= 192) UEPM_APPLY(12, BPL);     \
+    else if (n <= 256) UEPM_APPLY(16, BPL);     \
+    else if (n <= 384) UEPM_APPLY(24, BPL);     \
+    else if (n <= 576) UEPM_APPLY(36, BPL);     \
     else return hipErrorInvalidValue;
+    if (bpl >= 4) {
+        UEPM_APPLY_ROWS(4)
+    } else if (bpl == 2) {
+        UEPM_APPLY_ROWS(2)
+    } else {
+        UEPM_APPLY_ROWS(1)
+    }
+#undef UEPM_APPLY_ROWS
 #undef UEPM_APPLY
     return hipGetLastError();
 }
