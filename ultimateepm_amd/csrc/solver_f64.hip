@@ -18,6 +18,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <cstdlib>
 
 #include "epm_physics.hpp"
 #include "solver.hpp"
@@ -55,6 +56,41 @@ __device__ __forceinline__ double block_sum(double v, double* scratch) {
 }
 
 // ------------------------------------------------------------ (a) assembly
+// h_ij and h_ji of the pair (i, j) before hermitisation and spin-orbit: V(G),
+// the kinetic diagonal and the non-local term.  h_ji comes from h_ij where
+// that is exact: the phase argument of V(G) negates exactly under i <-> j and
+// the device sincos is exactly odd / even there; the non-local phase is
+// cos / sin of theta_i - theta_j formed from the per-vector cos / sin, which
+// swap to the exact conjugate (products commute, no FMA contraction); the
+// square-well radial integrals are symmetric in |K_i|, |K_j| outside the
+// near-equal branch of F_l (which reads one side's Bessel values) -- so one
+// evaluation gives the reference's two-sided 0.5 (h_ij + conj(h_ji)) bit for
+// bit.  Gaussian l = 2 wells and near-equal norms are evaluated both ways.
+__device__ __forceinline__ void pair_terms(const DevParams& P, const KState& si, const KState& sj, int gx, int gy,
+                                           int gz, bool diag, const double2* vtab, bool swap_exact, double& re_ij,
+                                           double& im_ij, double& re_ji, double& im_ji) {
+    local_V(P, gx, gy, gz, re_ij, im_ij, vtab);
+    re_ji = re_ij;
+    im_ji = -im_ij;
+    if (diag) {
+        const double KG2 = P.kin * (si.kx * si.kx + si.ky * si.ky + si.kz * si.kz);
+        re_ij += KG2;
+        re_ji += KG2;
+    }
+    if (P.nl) {
+        double r1, i1, r2, i2;
+        nonlocal(P, si, sj, r1, i1);
+        if (swap_exact && fabs(si.nK - sj.nK) > 1.0e-4) {
+            r2 = r1;
+            i2 = -i1;
+        } else {
+            nonlocal(P, sj, si, r2, i2);
+        }
+        re_ij += r1; im_ij += i1;
+        re_ji += r2; im_ji += i2;
+    }
+}
+
 // One workgroup per k.  Per-basis-vector state in LDS, then every (i, j) pair
 // of the lower triangle: the stored value is the reference's
 // 0.5 (h_ij + conj(h_ji)) (its Diagonalize hermitisation), mirrored into the
@@ -75,15 +111,7 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
     __syncthreads();
     double2* A = H + (size_t)kk * ld * P.dim;
     const int npairs = N * (N + 1) / 2;
-    // h_ji from h_ij: the phase argument of V(G) negates exactly under
-    // i <-> j and the device sincos is exactly odd / even there; the
-    // non-local phase is cos / sin of theta_i - theta_j formed from the
-    // per-vector cos / sin, which swap to the exact conjugate (products
-    // commute, no FMA contraction); the square-well radial integrals are symmetric in |K_i|,
-    // |K_j| outside the near-equal branch of F_l (which reads one side's
-    // Bessel values) -- so one evaluation gives the reference's two-sided
-    // 0.5 (h_ij + conj(h_ji)) bit for bit.  Gaussian l = 2 wells and
-    // near-equal norms are evaluated both ways.
+    // h_ji from h_ij where exact (pair_terms)
     const bool swap_exact = !(P.well != 0 && (P.need_l2_a || P.need_l2_c));
     // element t -> (i, j), column-major over the packed lower triangle; the
     // first from the closed form, then advanced by BS per iteration
@@ -105,26 +133,8 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
         const KState& si = ks[i];
         const KState& sj = ks[j];
         const int gx = G[3 * i] - G[3 * j], gy = G[3 * i + 1] - G[3 * j + 1], gz = G[3 * i + 2] - G[3 * j + 2];
-        double re_ij, im_ij;
-        local_V(P, gx, gy, gz, re_ij, im_ij, vtab);
-        double re_ji = re_ij, im_ji = -im_ij;
-        if (i == j) {
-            const double KG2 = P.kin * (si.kx * si.kx + si.ky * si.ky + si.kz * si.kz);
-            re_ij += KG2;
-            re_ji += KG2;
-        }
-        if (P.nl) {
-            double r1, i1, r2, i2;
-            nonlocal(P, si, sj, r1, i1);
-            if (swap_exact && fabs(si.nK - sj.nK) > 1.0e-4) {
-                r2 = r1;
-                i2 = -i1;
-            } else {
-                nonlocal(P, sj, si, r2, i2);
-            }
-            re_ij += r1; im_ij += i1;
-            re_ji += r2; im_ji += i2;
-        }
+        double re_ij, im_ij, re_ji, im_ji;
+        pair_terms(P, si, sj, gx, gy, gz, i == j, vtab, swap_exact, re_ij, im_ij, re_ji, im_ji);
         if (!P.soc) {
             const double hr = 0.5 * (re_ij + re_ji), hi = 0.5 * (im_ij - im_ji);
             A[i + (size_t)j * ld] = make_double2(hr, hi);
@@ -165,6 +175,71 @@ __global__ __launch_bounds__(BS) void assemble_kernel(DevParams P, const int32_t
                 A[i + (size_t)(j + n0) * ld] = make_double2(hr, -hi);
             }
         }
+    }
+}
+
+// Without spin-orbit: each wave takes 16 x 16 tiles (bi >= bj) of the lower
+// triangle, writes the tile and, through its LDS slot, the conjugate
+// transpose above the diagonal, so both halves go out in 256-byte column
+// runs (the mirrored writes of assemble_kernel are one element per column).
+// Element values are assemble_kernel's, bit for bit.
+template <int BS>
+__global__ __launch_bounds__(BS) void assemble_tiled_kernel(DevParams P, const int32_t* __restrict__ G,
+                                                             const double* __restrict__ kpts,
+                                                             double2* __restrict__ H, int ld) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    KState* ks = reinterpret_cast<KState*>(smem);
+    __shared__ double2 vtab[kVPhaseTab];
+    __shared__ double2 T[BS / 64][16][17];
+    const int kk = blockIdx.x;
+    const double* k = kpts + 3 * (size_t)kk;
+    const int N = P.N;
+    for (int i = threadIdx.x; i < N; i += BS) kstate_init(P, k, G, i, ks[i]);
+    for (int t = threadIdx.x; t < kVPhaseTab; t += BS) vtab[t] = local_phase(P, t / 49 - 3, (t / 7) % 7 - 3, t % 7 - 3);
+    __syncthreads();
+    double2* A = H + (size_t)kk * ld * P.dim;
+    const bool swap_exact = !(P.well != 0 && (P.need_l2_a || P.need_l2_c));
+    const int nt = (N + 15) >> 4, ntiles = nt * (nt + 1) / 2;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int il = lane & 15, jq = lane >> 4;
+    double2(*Tw)[17] = T[wv];
+    for (int tile = wv; tile < ntiles; tile += BS / 64) {
+        // tile -> (bi, bj), column-major over the lower triangle of tiles
+        int bj = 0, rem = tile;
+        while (rem >= nt - bj) {
+            rem -= nt - bj;
+            ++bj;
+        }
+        const int bi = bj + rem;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int jl = jq + 4 * q;
+            const int i = 16 * bi + il, j = 16 * bj + jl;
+            if (i < N && j < N && i >= j) {
+                const KState& si = ks[i];
+                const KState& sj = ks[j];
+                const int gx = G[3 * i] - G[3 * j], gy = G[3 * i + 1] - G[3 * j + 1], gz = G[3 * i + 2] - G[3 * j + 2];
+                double re_ij, im_ij, re_ji, im_ji;
+                pair_terms(P, si, sj, gx, gy, gz, i == j, vtab, swap_exact, re_ij, im_ij, re_ji, im_ji);
+                const double hr = 0.5 * (re_ij + re_ji), hi = 0.5 * (im_ij - im_ji);
+                // the diagonal as assemble_kernel's mirrored write leaves it
+                A[i + (size_t)j * ld] = make_double2(hr, i == j ? -hi : hi);
+                Tw[jl][il] = make_double2(hr, hi);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rl = il, cl = jq + 4 * q;
+            const int row = 16 * bj + rl, col = 16 * bi + cl;
+            if (row < N && col < N && row < col) {
+                const double2 h = Tw[rl][cl];  // h(col, row)
+                A[row + (size_t)col * ld] = make_double2(h.x, -h.y);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -619,16 +694,27 @@ __global__ __launch_bounds__(BS) void backtransform_kernel(const RT* __restrict_
 
 // ------------------------------------------------------------ launchers
 constexpr int kAsmBS = 256;
+// UEPM_ASM_V1=1 keeps the one-element-per-thread assembly for every plan (A/B)
+static bool assemble_v1() {
+    static const bool on = [] {
+        const char* v = std::getenv("UEPM_ASM_V1");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
 constexpr int kTriBS = 256;
 constexpr int kBtBS = 256;
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s) {
-    // one KState per basis vector in LDS beside the phase table: up to 860
-    // vectors in the 160 KB of a CU
+    // one KState per basis vector in LDS beside the phase table (and the tiled
+    // kernel's transpose slots): up to 766 vectors (860 with spin-orbit) in
+    // the 160 KB of a CU
     const size_t shm = sizeof(KState) * (size_t)P.N;
-    if (shm + sizeof(double2) * kVPhaseTab > 163840) return hipErrorInvalidValue;
-    auto kern = assemble_kernel<kAsmBS>;
+    const bool tiled = !P.soc && !assemble_v1();
+    const size_t fixed = sizeof(double2) * kVPhaseTab + (tiled ? sizeof(double2) * (kAsmBS / 64) * 16 * 17 : 0);
+    if (shm + fixed > 163840) return hipErrorInvalidValue;
+    auto kern = tiled ? assemble_tiled_kernel<kAsmBS> : assemble_kernel<kAsmBS>;
     hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(kAsmBS), shm, s, P, dG, dk, dH, ld);
