@@ -365,6 +365,14 @@ __device__ __forceinline__ double rcp_nr64(double x) {
     return fma(fma(-x, r, 1.0), r, r);
 }
 
+// 1/x from v_rcp_f64 and one Newton step: the Sturm recurrence is backward
+// stable, and an error of a few ulp in a pivot moves only the counts of
+// probes within rounding of an eigenvalue
+__device__ __forceinline__ double rcp_nr64_1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(fma(-x, r, 1.0), r, r);
+}
+
 // number of eigenvalues of T below x (Sturm count, pivmin-guarded)
 __device__ __forceinline__ int sturm_count_u(const double* __restrict__ d, const double* __restrict__ e, int n,
                                              double x, double pivmin) {
@@ -381,13 +389,13 @@ __device__ __forceinline__ int sturm_count_u(const double* __restrict__ d, const
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            q = (dd[u] - x) - (ee[u] * ee[u]) * rcp_nr64(q);
+            q = (dd[u] - x) - (ee[u] * ee[u]) * rcp_nr64_1(q);
             if (fabs(q) < pivmin) q = -pivmin;
             c += q < 0.0;
         }
     }
     for (; i < n; ++i) {
-        q = (d[i] - x) - (e[i - 1] * e[i - 1]) * rcp_nr64(q);
+        q = (d[i] - x) - (e[i - 1] * e[i - 1]) * rcp_nr64_1(q);
         if (fabs(q) < pivmin) q = -pivmin;
         c += q < 0.0;
     }
@@ -424,6 +432,29 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
     lo -= 2.0 * eps * tnorm + pivmin;
     hi += 2.0 * eps * tnorm + pivmin;
     const double abstol = abstol_rel * tnorm;
+    // two shared sweeps first: the wave's 64 lanes probe one interval that holds
+    // every band of the wave, and each band takes the probes around its index
+    // (a count c at x puts eigenvalues 0..c-1 below x and c.. at or above it)
+    {
+        const int bw0 = blockIdx.y * NBW;
+        const int bw1 = min(nb, bw0 + NBW) - 1;  // last band of this wave
+        double L = lo, U = hi;
+        for (int sw = 0; sw < 2; ++sw) {
+            const double x = L + (double)(lane + 1) * ((U - L) / 65.0);
+            const int cnt = sturm_count_u(d, e, n, x, pivmin);
+            double nlo = lo, nhi = hi;
+            for (int u = 0; u < 64; ++u) {
+                const int cu = __shfl(cnt, u, 64);
+                const double xu = __shfl(x, u, 64);
+                if (cu > b) nhi = fmin(nhi, xu);
+                else nlo = fmax(nlo, xu);
+            }
+            lo = nlo;
+            hi = nhi;
+            L = __shfl(lo, 0, 64);  // band bw0 sits on lane 0, band bw1 on lane (bw1 - bw0) S
+            U = __shfl(hi, (bw1 - bw0) * S, 64);
+        }
+    }
     // multisection: lane `sub` of a band probes lo + (sub+1) (hi-lo)/(S+1)
     for (int it = 0; it < 200; ++it) {
         const double w = hi - lo;
