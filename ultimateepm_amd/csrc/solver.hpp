@@ -37,8 +37,9 @@ hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, do
 // abstol_rel > 0 ends bisection at brackets narrower than abstol_rel ||T||_inf
 // (the precision form, whose T carries complex64 rounding); 0 bisects to the
 // rounding level of each eigenvalue.
+// inv_sweeps: inverse-iteration sweeps for the eigenvectors of T (0: eigenvalues only).
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
-                          double* dW, int want_vectors, double abstol_rel, hipStream_t s);
+                          double* dW, int inv_sweeps, double abstol_rel, hipStream_t s);
 hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2* dtau, const double* dd,
                                 const double* de, const double* devals, const double* dZ, double2* dX, double* dgrads,
                                 const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,

@@ -408,7 +408,7 @@ template <int S>
 __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __restrict__ dall,
                                                           const double* __restrict__ eall, int n, int nb, int64_t nk,
                                                           double* __restrict__ evals, double* __restrict__ Z,
-                                                          double* __restrict__ W, int want_vectors,
+                                                          double* __restrict__ W, int inv_sweeps,
                                                           double abstol_rel) {
     constexpr int NBW = 64 / S;  // bands per wave
     const int64_t kk = blockIdx.x;
@@ -482,11 +482,11 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
     }
     const double lam = 0.5 * (lo + hi);
     if (live && sub == 0) evals[kk * nb + b] = lam;
-    if (!want_vectors || !live || sub != 0) return;
+    if (inv_sweeps <= 0 || !live || sub != 0) return;
 
     // Inverse iteration with an LU factorisation of T - lam I with partial
-    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|), three sweeps
-    // from a fixed pseudo-random start.  Each forward sweep recomputes the
+    // pivoting (rows k, k+1 interchanged when |e_k| > |pivot|), inv_sweeps
+    // sweeps from a fixed pseudo-random start.  Each forward sweep recomputes the
     // multipliers and interchanges (the same operations, so the same values)
     // instead of reading them back, and each iterate's 1/max scaling is
     // applied when the next sweep reads it: the interleaved work arrays (over
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
     constexpr int PB = 4;
     const double tiny = eps * tnorm;
     double sc = 1.0;  // scaling of the iterate in X
-    for (int iter = 0; iter < 3; ++iter) {
+    for (int iter = 0; iter < inv_sweeps; ++iter) {
         const bool first = iter == 0;
         // forward: factor row k, apply the interchange and the multiplier
         double a = d[0] - lam;
@@ -760,16 +760,16 @@ hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, do
 }
 
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
-                          double* dW, int want_vectors, double abstol_rel, hipStream_t s) {
+                          double* dW, int inv_sweeps, double abstol_rel, hipStream_t s) {
     if (nb <= 16)
         hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, want_vectors, abstol_rel);
+                           dZ, dW, inv_sweeps, abstol_rel);
     else if (nb <= 32)
         hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, want_vectors, abstol_rel);
+                           dZ, dW, inv_sweeps, abstol_rel);
     else
         hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), 0, s, dd,
-                           de, n, nb, nk, devals, dZ, dW, want_vectors, abstol_rel);
+                           de, n, nb, nk, devals, dZ, dW, inv_sweeps, abstol_rel);
     return hipGetLastError();
 }
 

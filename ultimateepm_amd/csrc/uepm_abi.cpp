@@ -618,7 +618,8 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
             // for inverse iteration; the FP64 refinement sets the eigenvalues
             HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nbr, c, p->Ev.as<double>(),
-                                        p->Z.as<double>(), p->W.as<double>(), 1, kMixedBisectTol, s));
+                                        p->Z.as<double>(), p->W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
+                                        s));
             HIP_OK(rec(ci, kTriEig, 1));
             HIP_OK(rec(ci, kBack, 0));
             if (p->bt_v1) {
@@ -665,7 +666,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         HIP_OK(rec(ci, kTriEig, 0));
         HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nb, c, devals + c0 * nb,
                                     need_vec ? p->Z.as<double>() : nullptr, need_vec ? p->W.as<double>() : nullptr,
-                                    need_vec ? 1 : 0, 0.0, s));
+                                    need_vec ? kFp64InvSweeps : 0, 0.0, s));
         HIP_OK(rec(ci, kTriEig, 1));
         if (need_vec) {
             double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : p->X.as<double2>();

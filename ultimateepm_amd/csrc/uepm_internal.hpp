@@ -57,6 +57,11 @@ enum Stage { kAssemble = 0, kTridiag, kTriEig, kBack, kRefine, kNumStages };
 constexpr int kMixedExtra = 4;
 // Bisection bracket (relative to ||T||_inf) of the precision form's eigen-stage.
 constexpr double kMixedBisectTol = 1e-10;
+// Inverse-iteration sweeps: the FP64 form's vectors are its output; the
+// precision form's only seed the FP64 Rayleigh-Ritz step, whose values are
+// second order in their error (two sweeps leave ~1e-5 in the vectors)
+constexpr int kFp64InvSweeps = 3;
+constexpr int kMixedInvSweeps = 2;
 
 }  // namespace uepm_host
 
