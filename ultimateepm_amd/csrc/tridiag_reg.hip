@@ -584,9 +584,9 @@ hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int 
 }
 
 // ---- FP64: grids 192 (6 x 12, low-register, 256 VGPRs with a small spill), 160
-// (5 x 10, low-register), 128 (4 x 8), 96 (3 x 6), one workgroup per CU, and 64
-// (2 x 4, two).  A larger FP64 matrix runs the blocked reduction first and
-// hands over its trailing block at a panel boundary.
+// (5 x 10, low-register) and 128 (4 x 8), one workgroup per CU; 96 (3 x 6,
+// low-register, 128 VGPRs) and 64 (2 x 4), two.  A larger FP64 matrix runs the
+// blocked reduction first and hands over its trailing block at a panel boundary.
 namespace {
 constexpr int kNumGrids64 = 5;
 constexpr int kGrid64[kNumGrids64] = {192, 160, 128, 96, 64};
@@ -612,7 +612,7 @@ void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk
         case 0: reg_stage<double, double2, 6, 12, 1, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 1: reg_stage<double, double2, 5, 10, 1, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         case 2: reg_stage<double, double2, 4, 8, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<double, double2, 3, 6, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<double, double2, 3, 6, 4, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         default: reg_stage<double, double2, 2, 4, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
