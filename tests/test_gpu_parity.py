@@ -195,6 +195,28 @@ def test_mixed_precision_vs_oracle(uepm, oracle, gpu_ctx, sym, n, soc, nb):
         assert np.abs(pairs[:, :, 0] - pairs[:, :, 1]).max() <= MIXED_TOL
 
 
+@pytest.mark.parametrize("ng", [200, 240, 256, 259])
+def test_mixed_precision_wide_register_grids(uepm, oracle, gpu_ctx, ng):
+    """Precision form on the first ng vectors of the n = 14 basis: dims 200
+    and 240 start on the 224- and 256-wide register grids, 256 fills the
+    widest, and 259 (the whole basis) hands over from the blocked reduction."""
+    print(f"case: mixed Si non-local, first {ng} vectors of the n=14 basis", flush=True)
+    mat = _mat("Si")
+    G = np.ascontiguousarray(uepm.basis_vectors(14)[:ng])
+    plan = uepm.Plan(gpu_ctx, mat, G, uepm.NONLOCAL | uepm.MIXED | uepm.VECTORS, 12)
+    ks = np.vstack([[0.0, 0.0, 0.0], [0.5, 0.5, 0.5], uepm.bz_sample(77, 20)])
+    ev, vec, _ = plan.solve(ks)
+    w_o, _, _ = oracle.solve(mat, G, uepm.NONLOCAL, 12, ks, nthreads=8)
+    err = ev - w_o
+    print(f"dim {ng}: max |dE| = {np.abs(err).max() * 1e3:.3e} meV")
+    assert np.abs(err).max() <= MIXED_TOL
+    H = oracle.assemble(mat, G, ks[2], nl=True, soc=False)
+    X = vec[2].T
+    assert np.abs(X.conj().T @ X - np.eye(12)).max() <= 1e-10
+    assert np.linalg.norm(H @ X - X * ev[2], axis=0).max() <= 1e-3
+    plan.close()
+
+
 def test_mixed_precision_gradients_many_bands(uepm, oracle, gpu_ctx):
     """Precision form with 20 bands (24 Ritz vectors) and gradients against the FP64 oracle."""
     print("case: mixed Si n=8 nb=20 gradients", flush=True)

@@ -31,7 +31,7 @@ def test_config_validation_and_workspace(gpu_ctx):
     assert p.workspace_bytes(100) < p.workspace_bytes(512) == p.workspace_bytes(5000)
     q = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 274
     with pytest.raises(UepmError):
-        q.set_config(tridiag=2)  # register-resident reduction is for dim <= 192
+        q.set_config(tridiag=2)  # register-resident reduction is for dim <= 256
     p.close()
     q.close()
 

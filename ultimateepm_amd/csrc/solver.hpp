@@ -68,9 +68,9 @@ int tridiag_reg_split(int n);
 size_t tridiag_reg_work_bytes(int n);
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s);
-// Complex64 register grids after the blocked reduction for n > 192: the
+// Complex64 register grids after the blocked reduction for n > 256: the
 // trailing size they take over (a panel boundary of the blocked kernel; n
-// itself for n <= 192; 0 with UEPM_REG_SPLIT=0) and the launch on a complex64
+// itself for n <= 256; 0 with UEPM_REG_SPLIT=0) and the launch on a complex64
 // block left in place by the blocked kernel (arguments as for the FP64 form
 // below; the hand-over workspace is tridiag_reg_work_bytes(m)).
 int tridiag_reg_c64_tail(int n);

@@ -1,7 +1,8 @@
 // tridiag_reg.hip — register-resident Householder reduction of a Hermitian
 // H(k) to real symmetric tridiagonal form, one 512-lane workgroup per
-// k-point, for dim <= 192 (gfx950): complex64 arithmetic for the precision
-// form, complex128 for the FP64 form, and a chain of shrinking grids (below).
+// k-point (gfx950): complex64 arithmetic for the precision form (dim <= 256),
+// complex128 for the FP64 form (dim <= 192), and a chain of shrinking grids
+// (below).
 //
 // Layout.  The 512 lanes form a PR x PC = 32 x 16 grid: lane (r, c) with
 // c = tid % 16, r = tid / 16 (a wave holds four r values and all sixteen c).
@@ -465,15 +466,17 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     }
 }
 
-// n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12 (rows x columns of the 32 x 16 grid)
-int tridiag_reg_max_n() { return 192; }
+// n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12; n <= 256: 8 x 16
+// (rows x columns of the 32 x 16 grid)
+int tridiag_reg_max_n() { return 256; }
 
 // Hand-over between grids.  The late steps of a large grid carry its full
 // per-step overhead for a small trailing matrix, so each grid stops where the
-// next smaller one can take the trailing block: 192-grid (6 x 12 blocks) ->
-// 160 (5 x 10) -> 128 (4 x 8) -> 96 (3 x 6) -> 64 (2 x 4).  Register budgets
-// (workgroups per CU): 192 and 160 low-register, 128 VGPRs (2); 128 at 128
-// VGPRs (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
+// next smaller one can take the trailing block: 256-grid (8 x 16 blocks) ->
+// 224 (7 x 14) -> 192 (6 x 12) -> 160 (5 x 10) -> 128 (4 x 8) -> 96 (3 x 6) ->
+// 64 (2 x 4).  Register budgets (workgroups per CU): 256 and 224
+// low-register, 256 and 223 VGPRs (1); 192 and 160 low-register, 128 VGPRs
+// (2); 128 at 128 VGPRs (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
 // UEPM_REG_SPLIT=0 keeps one grid.
 namespace {
 int reg_split_mode() {
@@ -483,8 +486,8 @@ int reg_split_mode() {
     }();
     return mode;
 }
-constexpr int kNumGrids = 5;
-constexpr int kGrid[kNumGrids] = {192, 160, 128, 96, 64};
+constexpr int kNumGrids = 7;
+constexpr int kGrid[kNumGrids] = {256, 224, 192, 160, 128, 96, 64};
 // index of the smallest grid that holds n
 int grid_of(int n) {
     int g = 0;
@@ -519,10 +522,12 @@ template <typename THin>
 void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk,
                 double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
     switch (g) {
-        case 0: reg_stage<float, float2, 6, 12, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 1: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 2: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<float, float2, 3, 6, 6, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 0: reg_stage<float, float2, 8, 16, 2, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<float, float2, 7, 14, 2, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 2: reg_stage<float, float2, 6, 12, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 4: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 5: reg_stage<float, float2, 3, 6, 6, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
         default: reg_stage<float, float2, 2, 4, 8, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
@@ -563,7 +568,7 @@ hipError_t c64_chain(const THin* H, int64_t hk, int m, int ld, int64_t nk, float
 
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s) {
-    if (n < 1 || n > 192 || nk <= 0) return hipErrorInvalidValue;
+    if (n < 1 || n > kGrid[0] || nk <= 0) return hipErrorInvalidValue;
     const int64_t hk = (int64_t)ld * n;
     return c64_chain(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, work, s);
 }

@@ -53,7 +53,7 @@ int64_t plan_chunk(const uepm_plan* p, int64_t nk) {
 }
 
 namespace {
-// Trailing size of the precision form's blocked reduction (n > 192) that the
+// Trailing size of the precision form's blocked reduction (n > 256) that the
 // register grids take over; 0 when the configuration asks for the blocked kernel.
 int mixed_c64_tail(const uepm_plan* p) {
     if (p->tri_reg || p->cfg.tridiag == 1 || p->dim <= uepm::tridiag_reg_max_n()) return 0;
@@ -469,10 +469,15 @@ int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, ue
     hipStream_t s = (hipStream_t)stream_;
     // candidates: reduction variant x workgroup size, back-transform band groups
     std::vector<uepm_config> cand;
-    const bool reg_ok = (p->flags & UEPM_MIXED) && p->dim <= uepm::tridiag_reg_max_n();
+    const bool mixed = p->flags & UEPM_MIXED;
+    const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
     for (int bt : {8, 16}) {
         if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk});
-        for (int tb : {256, 512}) cand.push_back({1, tb, bt, p->cfg.max_chunk});
+        for (int tb : {256, 512}) {
+            cand.push_back({1, tb, bt, p->cfg.max_chunk});
+            // above the register grids: blocked leading steps, register-grid tail
+            if (mixed && !reg_ok) cand.push_back({0, tb, bt, p->cfg.max_chunk});
+        }
     }
     const uepm_config saved = p->cfg;
     const int saved_timing = p->timing;
@@ -598,7 +603,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
                 HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, p->d.as<double>(), p->e.as<double>(),
                                                     p->tauf.as<float2>(), p->S32.as<float2>(), s));
             else {
-                // n > 192: the blocked reduction for the leading steps, the register grids
+                // n > 256: the blocked reduction for the leading steps, the register grids
                 // for the trailing block it leaves in place in Vc
                 const int t = mixed_c64_tail(p);
                 const int s0 = n - t;
