@@ -7,12 +7,13 @@
 //                        T-eigenvectors inside clusters of close eigenvalues
 //                        (inverse iteration leaves them only nearly orthogonal),
 //                        then unit norm.
-//   apply_q_kernel       one 16-lane DPP row per band: lane rs holds rows
-//                        rs + 16 q in registers, so the dot v_j^H x of every
-//                        reflector is a 16-lane DPP sum; no LDS and no barriers.
-//                        Reflector columns are read from the reduction's output
-//                        (L2-resident, shared by the bands of a k-point) one step
-//                        ahead of their use.
+//   apply_q_kernel       one 16-lane DPP row per 1, 2 or 4 bands: lane rs holds
+//                        rows rs + 16 q of its bands in registers, so the dot
+//                        v_j^H x of every reflector is a 16-lane DPP sum (no LDS,
+//                        no barriers) and each reflector row a lane loads serves
+//                        all of its bands.  Reflector columns are read from the
+//                        reduction's output (L2-resident, shared by the bands of
+//                        a k-point) one step ahead of their use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
