@@ -49,6 +49,26 @@ def test_tuned_plan_keeps_results(gpu_ctx):
     p.close()
 
 
+def test_tuned_plan_above_the_register_grids(gpu_ctx):
+    """Above dim 256 the tuner chooses between the LDS-blocked reduction alone
+    and its leading steps followed by the register grids; every candidate
+    gives the same eigenvalues to the precision form's bar."""
+    print("case: autotune Ge SOC n=10 mixed (dim 274), 256 k", flush=True)
+    ks = api.bz_sample(11, 256)
+    p = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)
+    assert p.dim == 274
+    ev0, _, _ = p.solve(ks)
+    dk = _dev(gpu_ctx, ks)
+    chosen = p.tune(dk.ptr, len(ks))
+    assert chosen["tridiag"] in (0, 1) and chosen["tri_block"] in (256, 512)
+    ev1, _, _ = p.solve(ks)
+    assert np.abs(ev1 - ev0).max() <= 1e-8
+    p.set_config(tridiag=1, tri_block=512, bt_bands=8, max_chunk=0)  # blocked only
+    ev2, _, _ = p.solve(ks)
+    assert np.abs(ev2 - ev0).max() <= 1e-8
+    p.close()
+
+
 def test_batch_graph_matches_individual_solves(gpu_ctx):
     print("case: batch graph over 4 uneven plans, 2 replays", flush=True)
     specs = [("Si", 8, api.NONLOCAL | api.MIXED, 12), ("Ge", 8, api.NONLOCAL | api.SOC | api.MIXED, 16),
