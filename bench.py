@@ -436,8 +436,8 @@ def main() -> int:
         return copy_calibration(args.copy_calibration, 10)
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU: the k-point batch shards across GPUs by "
-                                   "block partition (see DESIGN.md), but no multi-GPU run is part of it",
+        print(json.dumps({"error": "one GPU only: the k-point batch would shard across GPUs by block "
+                                   "partition, one process per GPU, but no multi-GPU run is implemented",
                           "n_gpus": args.gpus}))
         return 2
 
