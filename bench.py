@@ -128,6 +128,44 @@ def serving_leg(ctx, requests: int, seed: int = SEED, budget_gib: int = 64, infl
     return out
 
 
+def dims_leg(ctx, seed: int = SEED) -> list:
+    """Throughput by basis dimension: Si (n = 6..14 shells, dims 59..259) and Ge with
+    spin-orbit (n = 6..14, dims 118..518), non-local, 16 bands, eigenvalues only,
+    2048 seeded BZ k-points (512 above dim 200), both forms; one untimed solve,
+    then one timed solve (wall clock over a device-resident batch) and the stage
+    split of a third."""
+    from ultimateepm_amd import api
+    from ultimateepm_amd.material import load
+
+    rows = []
+    for sym, soc in (("Si", False), ("Ge", True)):
+        for n in (6, 8, 10, 12, 14):
+            for mode in ("fp64", "mixed"):
+                flags = api.NONLOCAL | (api.SOC if soc else 0) | (api.MIXED if mode == "mixed" else 0)
+                nb = 16
+                p = api.Plan(ctx, load(sym), api.basis_vectors(n), flags, nb)
+                nk = 2048 if p.dim <= 200 else 512
+                dk = api.DeviceArray.from_host(ctx, api.bz_sample(seed + 5, nk))
+                de = api.DeviceArray(ctx, (nk, nb))
+                p.solve_device(dk.ptr, nk, de.ptr)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                p.solve_device(dk.ptr, nk, de.ptr)
+                ctx.synchronize()
+                dt = time.perf_counter() - t0
+                p.set_timing(True)
+                p.solve_device(dk.ptr, nk, de.ptr)
+                ctx.synchronize()
+                rows.append({"material": sym, "shells": n, "soc": soc, "dim": p.dim, "mode": mode, "nk": nk,
+                             "kpts_per_s": round(nk / dt, 1),
+                             "stage_ms": {k: round(v[0], 3) for k, v in p.last_timing().items()}})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+                p.close()
+                dk.free()
+                de.free()
+    return rows
+
+
 def batching_leg(ctx, nk_per_item: int, branches: int = 8, ws_cap_gib: float = 2.0, seed: int = SEED) -> dict:
     """Batching form: {Si, Ge, GaAs, InP} x n in {6, 8, 10, 12} shells x SOC off/on
     (InP + SOC excluded: its anion period has no spin-orbit radial model), nk
@@ -417,8 +455,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
-    ap.add_argument("--form", default="throughput", choices=["throughput", "serving", "batching"],
-                    help="throughput: the headline batched solve (default); serving / batching: those forms alone")
+    ap.add_argument("--form", default="throughput", choices=["throughput", "serving", "batching", "dims"],
+                    help="throughput: the headline batched solve (default); serving / batching: those forms "
+                         "alone; dims: throughput by basis dimension")
     ap.add_argument("--requests", type=int, default=200, help="requests of --form serving")
     ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
     ap.add_argument("--serving-requests", type=int, default=100,
@@ -444,6 +483,19 @@ def main() -> int:
     from ultimateepm_amd import api
     from ultimateepm_amd.material import load
 
+    if args.form == "dims":
+        ctx = api.Context(0)
+        rows = dims_leg(ctx)
+        head = next(r for r in rows if r["dim"] == 181 and r["mode"] == "mixed")
+        line = {"metric": "k-pts/s by basis dimension (16 bands, eigenvalues only); value: Si n=12 (dim 181) precision form",
+                "value": head["kpts_per_s"], "unit": "k-pts/s", "n_gpus": 1, "higher_is_better": True,
+                "data": "synthetic", "dtype": "c64+f64",
+                "config": {"workload": "Si n=6..14 and Ge+SOC n=6..14, non-local, nb 16, seeded BZ k-points",
+                           "form": "dims"},
+                "dims": rows}
+        print(json.dumps(line), flush=True)
+        ctx.close()
+        return 0
     if args.form == "batching":
         ctx = api.Context(0)
         bt = batching_leg(ctx, args.batch_nk)
