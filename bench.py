@@ -455,6 +455,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
+    ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2],
+                    help="chunks in flight (plan configuration; 0 = automatic: 2 for a multi-chunk solve)")
     ap.add_argument("--form", default="throughput", choices=["throughput", "serving", "batching", "dims"],
                     help="throughput: the headline batched solve (default); serving / batching: those forms "
                          "alone; dims: throughput by basis dimension")
@@ -531,6 +533,10 @@ def main() -> int:
     def run(mode: str):
         flags = api.NONLOCAL | api.VECTORS | (api.MIXED if mode == "mixed" else 0)
         plan = api.Plan(ctx, mat, G, flags, nb)
+        plan.set_config(lanes=1)
+        one = plan.workspace_bytes(args.nk)
+        plan.set_config(lanes=args.lanes)
+        lanes = plan.workspace_bytes(args.nk) // one  # chunks in flight
         dev = api.DeviceArray(ctx, (args.nk, nb))
         dvec = api.DeviceArray(ctx, (args.nk, nb, plan.dim, 2))
 
@@ -557,7 +563,7 @@ def main() -> int:
         plan.close()
         dev.free()
         dvec.free()
-        return {"elapsed": elapsed, "stage_ms": stage_ms, "evals": evals, "dim": plan.dim}
+        return {"elapsed": elapsed, "stage_ms": stage_ms, "evals": evals, "dim": plan.dim, "lanes": lanes}
 
     res = {m: run(m) for m in args.modes.split(",")}
     head = "mixed" if "mixed" in res else "fp64"
@@ -586,6 +592,14 @@ def main() -> int:
         "k_per_launch": round(args.nk * args.steps / max(dom_launches, 1), 1),
         "flop_per_k": tridiag_flops(r["dim"]),
         "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stage_ms.items()},
+        # Two lanes run alternate chunks through the same stage sequence side by
+        # side, so each launch shares the chip with the other lane's launch of the
+        # same kernel: the chip-level rate is lanes x the per-launch rate (check:
+        # the stage sum over lanes is about ms_per_step).
+        "launches_in_flight": r["lanes"],
+        "chip_achieved": round(achieved * r["lanes"], 3),
+        "chip_frac": round(achieved * r["lanes"] / peak, 4),
+        "stage_sum_over_lanes_ms": round(sum(v[0] for v in stage_ms.values()) / args.steps / r["lanes"], 3),
     }
     bpk = measured_traffic(head, dominant, N)
     if bpk is not None:

@@ -163,6 +163,9 @@ typedef struct uepm_config {
     int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = auto) */
     int32_t bt_bands;  /* bands per back-transform workgroup: 4, 8 or 16 (0 = 16)    */
     int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */
+    int32_t lanes;     /* chunks in flight: 0 automatic (2 when a solve spans two or
+                          more chunks: alternate chunks on the caller's stream and on
+                          a plan-owned stream, twice the workspace); 1 one chunk    */
 } uepm_config;
 int uepm_plan_get_config(const uepm_plan* plan, uepm_config* cfg);
 /* Values outside the ranges above -> UEPM_ERR_INVALID_ARG (plan unchanged). */

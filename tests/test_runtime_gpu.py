@@ -21,19 +21,59 @@ def _dev(ctx, a):
 def test_config_validation_and_workspace(gpu_ctx):
     print("case: config round trip Si n=12 mixed", flush=True)
     p = api.Plan(gpu_ctx, load("Si"), api.basis_vectors(12), api.NONLOCAL | api.MIXED, 12)
-    assert p.config() == {"tridiag": 0, "tri_block": 0, "bt_bands": 0, "max_chunk": 0}
+    assert p.config() == {"tridiag": 0, "tri_block": 0, "bt_bands": 0, "max_chunk": 0, "lanes": 0}
     p.set_config(tridiag=1, tri_block=256, bt_bands=8, max_chunk=512)
-    assert p.config() == {"tridiag": 1, "tri_block": 256, "bt_bands": 8, "max_chunk": 512}
-    for bad in ({"tridiag": 3}, {"tri_block": 128}, {"bt_bands": 5}, {"max_chunk": -1}):
+    assert p.config() == {"tridiag": 1, "tri_block": 256, "bt_bands": 8, "max_chunk": 512, "lanes": 0}
+    for bad in ({"tridiag": 3}, {"tri_block": 128}, {"bt_bands": 5}, {"max_chunk": -1}, {"lanes": 3}):
         with pytest.raises(UepmError):
             p.set_config(**bad)
     assert p.config()["max_chunk"] == 512  # unchanged by a rejected config
-    assert p.workspace_bytes(100) < p.workspace_bytes(512) == p.workspace_bytes(5000)
+    # one chunk in flight up to max_chunk points, two above it
+    assert p.workspace_bytes(100) < p.workspace_bytes(512) and p.workspace_bytes(5000) == 2 * p.workspace_bytes(512)
+    p.set_config(tridiag=1, tri_block=256, bt_bands=8, max_chunk=512, lanes=1)
+    assert p.workspace_bytes(5000) == p.workspace_bytes(512)
     q = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 274
     with pytest.raises(UepmError):
         q.set_config(tridiag=2)  # register-resident reduction is for dim <= 256
     p.close()
     q.close()
+
+
+@pytest.mark.parametrize("flags", [api.NONLOCAL | api.VECTORS, api.NONLOCAL | api.VECTORS | api.MIXED])
+def test_two_lanes_equal_one_lane(gpu_ctx, flags):
+    """A solve over several chunks runs alternate chunks on a second stream with
+    its own workspace; every k-point's result is bit-identical to the one-lane
+    solve, on the default stream and on a caller's stream, with stage timing on."""
+    print(f"case: lanes 2 vs 1, Si n=10 flags {flags}, 1100 k in chunks of 256", flush=True)
+    ks = api.bz_sample(13, 1100)
+    nb = 10
+    p = api.Plan(gpu_ctx, load("Si"), api.basis_vectors(10), flags, nb)
+    dk = _dev(gpu_ctx, ks)
+    out = {}
+    stream = gpu_ctx.stream()
+    for lanes, sh, timing in ((1, 0, False), (2, 0, False), (2, stream.handle, True), (0, stream.handle, False)):
+        p.set_config(max_chunk=256, lanes=lanes)
+        p.set_timing(timing)
+        de = api.DeviceArray(gpu_ctx, (len(ks), nb))
+        dv = api.DeviceArray(gpu_ctx, (len(ks), nb, p.dim, 2))
+        de.zero()
+        dv.zero()
+        gpu_ctx.synchronize()
+        p.solve_device(dk.ptr, len(ks), de.ptr, dv.ptr, 0, sh)
+        if sh:
+            stream.synchronize()  # the caller's stream alone covers both lanes
+        e, v = de.numpy(), dv.numpy()
+        if timing:
+            assert all(n == 5 for _, n in p.last_timing().values() if n)  # ceil(1100 / 256) launches
+        if out:
+            assert np.array_equal(e, out["e"]) and np.array_equal(v, out["v"])
+        else:
+            out = {"e": e, "v": v}
+            assert np.all(np.isfinite(e)) and np.all(np.diff(e, axis=1) >= 0)
+        de.free()
+        dv.free()
+    stream.close()
+    p.close()
 
 
 def test_tuned_plan_keeps_results(gpu_ctx):

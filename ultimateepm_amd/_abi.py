@@ -59,7 +59,8 @@ class uepm_material(C.Structure):
 class uepm_config(C.Structure):
     """Mirror of ``uepm_config``: 0 fields mean automatic."""
 
-    _fields_ = [("tridiag", C.c_int32), ("tri_block", C.c_int32), ("bt_bands", C.c_int32), ("max_chunk", C.c_int32)]
+    _fields_ = [("tridiag", C.c_int32), ("tri_block", C.c_int32), ("bt_bands", C.c_int32), ("max_chunk", C.c_int32),
+                ("lanes", C.c_int32)]
 
 
 class uepm_batch_item(C.Structure):

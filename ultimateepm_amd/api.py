@@ -193,17 +193,19 @@ class Plan:
     def config(self) -> dict:
         c = _abi.uepm_config()
         check(lib().uepm_plan_get_config(self._h, C.byref(c)))
-        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk}
+        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk,
+                "lanes": c.lanes}
 
-    def set_config(self, tridiag: int = 0, tri_block: int = 0, bt_bands: int = 0, max_chunk: int = 0):
-        c = _abi.uepm_config(tridiag, tri_block, bt_bands, max_chunk)
+    def set_config(self, tridiag: int = 0, tri_block: int = 0, bt_bands: int = 0, max_chunk: int = 0, lanes: int = 0):
+        c = _abi.uepm_config(tridiag, tri_block, bt_bands, max_chunk, lanes)
         check(lib().uepm_plan_set_config(self._h, C.byref(c)))
 
     def tune(self, k_ptr: int, nk: int, stream: int = 0) -> dict:
         """Autotuner: times the candidate configurations on device k-points, keeps the fastest."""
         c = _abi.uepm_config()
         check(lib().uepm_plan_tune(self._h, C.c_void_p(k_ptr), nk, C.c_void_p(stream), C.byref(c)))
-        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk}
+        return {"tridiag": c.tridiag, "tri_block": c.tri_block, "bt_bands": c.bt_bands, "max_chunk": c.max_chunk,
+                "lanes": c.lanes}
 
     def workspace_bytes(self, nk: int) -> int:
         b = C.c_uint64(0)

@@ -77,7 +77,8 @@ int uepm_batch_create(uepm_ctx* ctx, const uepm_batch_item* items, int nitems, i
         double cost = 0.0;
         for (int i : j.second) cost += (double)items[i].nk * j.first->dim * j.first->dim * j.first->dim;
         order.push_back({cost, j.first});
-        const int st = uepm_plan_reserve(j.first, maxnk[j.first]);
+        // solves under capture run one lane
+        const int st = plan_reserve(j.first, plan_chunk(j.first, maxnk[j.first]), 1);
         if (st != UEPM_OK) return st;
     }
     std::sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
@@ -269,6 +270,7 @@ PooledPlan* acquire_plan(uepm_server* S, const Request& r, const PlanKey& key, i
         const uint64_t share = std::max<uint64_t>(S->budget / (uint64_t)std::max(1, S->max_inflight), per_k);
         uepm_config cfg{};
         cfg.max_chunk = (int32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, share / per_k));
+        cfg.lanes = 1;  // the slots are the concurrency; one request keeps one workspace
         uepm_plan_set_config(plan, &cfg);
         S->plans.push_back(PooledPlan{plan, key, 0, 0, Clock::now()});
         pp = &S->plans.back();

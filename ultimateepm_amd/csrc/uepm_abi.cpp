@@ -98,27 +98,33 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
 }
 }  // namespace
 
-size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk) {
+int plan_lanes(const uepm_plan* p, int64_t nk) { return (p->cfg.lanes == 1 || nk <= plan_chunk(p, nk)) ? 1 : 2; }
+
+size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk, int lanes) {
     const Sizes z = plan_sizes(p, chunk);
-    return z.H + z.d + z.e + z.tau + z.Vf + z.tauf + z.S32 + z.Ev + z.X0 + z.Y + z.S + z.Z + z.W + z.X;
+    return (size_t)lanes *
+           (z.H + z.d + z.e + z.tau + z.Vf + z.tauf + z.S32 + z.Ev + z.X0 + z.Y + z.S + z.Z + z.W + z.X);
 }
 
-int plan_reserve(uepm_plan* p, int64_t chunk) {
+int plan_reserve(uepm_plan* p, int64_t chunk, int lanes) {
     const Sizes z = plan_sizes(p, chunk);
-    HIP_OK(p->H.ensure(z.H));
-    HIP_OK(p->d.ensure(z.d));
-    HIP_OK(p->e.ensure(z.e));
-    HIP_OK(p->tau.ensure(z.tau));
-    if (z.Vf) HIP_OK(p->Vf.ensure(z.Vf));
-    if (z.tauf) HIP_OK(p->tauf.ensure(z.tauf));
-    if (z.S32) HIP_OK(p->S32.ensure(z.S32));
-    if (z.Ev) HIP_OK(p->Ev.ensure(z.Ev));
-    if (z.X0) HIP_OK(p->X0.ensure(z.X0));
-    if (z.Y) HIP_OK(p->Y.ensure(z.Y));
-    if (z.S) HIP_OK(p->S.ensure(z.S));
-    if (z.Z) HIP_OK(p->Z.ensure(z.Z));
-    if (z.W) HIP_OK(p->W.ensure(z.W));
-    if (z.X) HIP_OK(p->X.ensure(z.X));
+    for (int l = 0; l < lanes; ++l) {
+        Lane& w = p->lane[l];
+        HIP_OK(w.H.ensure(z.H));
+        HIP_OK(w.d.ensure(z.d));
+        HIP_OK(w.e.ensure(z.e));
+        HIP_OK(w.tau.ensure(z.tau));
+        if (z.Vf) HIP_OK(w.Vf.ensure(z.Vf));
+        if (z.tauf) HIP_OK(w.tauf.ensure(z.tauf));
+        if (z.S32) HIP_OK(w.S32.ensure(z.S32));
+        if (z.Ev) HIP_OK(w.Ev.ensure(z.Ev));
+        if (z.X0) HIP_OK(w.X0.ensure(z.X0));
+        if (z.Y) HIP_OK(w.Y.ensure(z.Y));
+        if (z.S) HIP_OK(w.S.ensure(z.S));
+        if (z.Z) HIP_OK(w.Z.ensure(z.Z));
+        if (z.W) HIP_OK(w.W.ensure(z.W));
+        if (z.X) HIP_OK(w.X.ensure(z.X));
+    }
     return UEPM_OK;
 }
 
@@ -443,6 +449,7 @@ int uepm_plan_set_config(uepm_plan* p, const uepm_config* cfg) {
     if (cfg->bt_bands != 0 && cfg->bt_bands != 4 && cfg->bt_bands != 8 && cfg->bt_bands != 16)
         return fail(UEPM_ERR_INVALID_ARG, "uepm_config.bt_bands must be 0, 4, 8 or 16");
     if (cfg->max_chunk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_config.max_chunk must be >= 0");
+    if (cfg->lanes < 0 || cfg->lanes > 2) return fail(UEPM_ERR_INVALID_ARG, "uepm_config.lanes must be 0, 1 or 2");
     p->cfg = *cfg;
     plan_apply_config(p);
     return UEPM_OK;
@@ -451,7 +458,7 @@ int uepm_plan_set_config(uepm_plan* p, const uepm_config* cfg) {
 int uepm_plan_workspace_bytes(const uepm_plan* p, int64_t nk, uint64_t* bytes) {
     if (!p || !bytes) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_workspace_bytes: null argument");
     if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "nk < 0");
-    *bytes = nk == 0 ? 0 : (uint64_t)plan_workspace_bytes(p, plan_chunk(p, nk));
+    *bytes = nk == 0 ? 0 : (uint64_t)plan_workspace_bytes(p, plan_chunk(p, nk), plan_lanes(p, nk));
     return UEPM_OK;
 }
 
@@ -460,7 +467,7 @@ int uepm_plan_reserve(uepm_plan* p, int64_t nk) {
     if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "nk < 0");
     if (nk == 0) return UEPM_OK;
     HIP_OK(hipSetDevice(p->ctx->device));
-    return plan_reserve(p, plan_chunk(p, nk));
+    return plan_reserve(p, plan_chunk(p, nk), plan_lanes(p, nk));
 }
 
 int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, uepm_config* chosen) {
@@ -472,11 +479,11 @@ int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, ue
     const bool mixed = p->flags & UEPM_MIXED;
     const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
     for (int bt : {8, 16}) {
-        if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk});
+        if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk, p->cfg.lanes});
         for (int tb : {256, 512}) {
-            cand.push_back({1, tb, bt, p->cfg.max_chunk});
+            cand.push_back({1, tb, bt, p->cfg.max_chunk, p->cfg.lanes});
             // above the register grids: blocked leading steps, register-grid tail
-            if (mixed && !reg_ok) cand.push_back({0, tb, bt, p->cfg.max_chunk});
+            if (mixed && !reg_ok) cand.push_back({0, tb, bt, p->cfg.max_chunk, p->cfg.lanes});
         }
     }
     const uepm_config saved = p->cfg;
@@ -557,11 +564,25 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
     const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
     const int64_t chunk = plan_chunk(p, nk);
+    int lanes = plan_lanes(p, nk);
+    if (lanes > 1) {  // a stream under graph capture keeps one lane
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) lanes = 1;
+    }
     {
-        const int st = plan_reserve(p, chunk);
+        const int st = plan_reserve(p, chunk, lanes);
         if (st != UEPM_OK) return st;
     }
     const int nchunks = (int)((nk + chunk - 1) / chunk);
+    if (lanes > 1) {
+        // the side stream starts after the caller's earlier work and the caller's
+        // stream resumes after the side stream's last chunk (join below)
+        if (!p->side) HIP_OK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        if (!p->fork) HIP_OK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+        if (!p->join) HIP_OK(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(p->fork, s));
+        HIP_OK(hipStreamWaitEvent(p->side, p->fork, 0));
+    }
     if (p->timing) {
         const size_t need = (size_t)nchunks * kNumStages * 2;
         while (p->ev.size() < need) {
@@ -572,7 +593,8 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     }
     auto rec = [&](int chunk_i, int stage, int end) -> hipError_t {
         if (!p->timing) return hipSuccess;
-        return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end], s);
+        return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end],
+                              chunk_i % lanes ? p->side : s);
     };
     bool used[kNumStages] = {true, true, true, stage_vec, mixed};
     p->stage_name[kTridiag] = mixed ? (p->tri_reg              ? "tridiag_reg_kernel<c64>"
@@ -588,68 +610,70 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
     for (int ci = 0; ci < nchunks; ++ci) {
         const int64_t c0 = (int64_t)ci * chunk;
+        Lane& w = p->lane[ci % lanes];
+        const hipStream_t cs = ci % lanes ? p->side : s;
         const int64_t c = std::min(chunk, nk - c0);
         const double* kc = dk + 3 * c0;
-        double2* Hc = p->H.as<double2>();
+        double2* Hc = w.H.as<double2>();
         HIP_OK(rec(ci, kAssemble, 0));
-        HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, s));
+        HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
         HIP_OK(rec(ci, kAssemble, 1));
         if (mixed) {
             // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
             // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
-            float2* Vc = p->Vf.as<float2>();
+            float2* Vc = w.Vf.as<float2>();
             HIP_OK(rec(ci, kTridiag, 0));
             if (p->tri_reg)
-                HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, p->d.as<double>(), p->e.as<double>(),
-                                                    p->tauf.as<float2>(), p->S32.as<float2>(), s));
+                HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d.as<double>(), w.e.as<double>(),
+                                                    w.tauf.as<float2>(), w.S32.as<float2>(), cs));
             else {
                 // n > 256: the blocked reduction for the leading steps, the register grids
                 // for the trailing block it leaves in place in Vc
                 const int t = mixed_c64_tail(p);
                 const int s0 = n - t;
-                HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, p->S32.p, Vc, p->d.as<double>(),
-                                                        p->e.as<double>(), p->tauf.as<float2>(), s,
+                HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, w.S32.p, Vc, w.d.as<double>(),
+                                                        w.e.as<double>(), w.tauf.as<float2>(), cs,
                                                         t > 0 ? s0 : n - 1));
                 if (t > 0) {
                     const int64_t nn = (int64_t)n * n;
                     float2* B = Vc + s0 + (int64_t)s0 * n;
-                    HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, p->d.as<double>() + s0,
-                                                              p->e.as<double>() + s0, p->tauf.as<float2>() + s0, n,
-                                                              p->S32.as<float2>(), s));
+                    HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
+                                                              w.e.as<double>() + s0, w.tauf.as<float2>() + s0, n,
+                                                              w.S32.as<float2>(), cs));
                 }
             }
             HIP_OK(rec(ci, kTridiag, 1));
             HIP_OK(rec(ci, kTriEig, 0));
             // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
             // for inverse iteration; the FP64 refinement sets the eigenvalues
-            HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nbr, c, p->Ev.as<double>(),
-                                        p->Z.as<double>(), p->W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
-                                        s));
+            HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nbr, c, w.Ev.as<double>(),
+                                        w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
+                                        cs));
             HIP_OK(rec(ci, kTriEig, 1));
             HIP_OK(rec(ci, kBack, 0));
             if (p->bt_v1) {
-                HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, p->tauf.as<float2>(), p->d.as<double>(),
-                                                      p->e.as<double>(), p->Ev.as<double>(), p->Z.as<double>(),
-                                                      p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, s));
+                HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, w.tauf.as<float2>(), w.d.as<double>(),
+                                                      w.e.as<double>(), w.Ev.as<double>(), w.Z.as<double>(),
+                                                      w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, cs));
             } else {
-                HIP_OK(uepm::launch_cluster_orth(p->d.as<double>(), p->e.as<double>(), p->Ev.as<double>(),
-                                                 p->Z.as<double>(), n, nbr, c, s));
-                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, p->tauf.as<float2>(), p->Z.as<double>(), nbr,
-                                                p->X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
-                                                s));
+                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
+                                                 w.Z.as<double>(), n, nbr, c, cs));
+                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
+                                                w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
+                                                cs));
             }
             HIP_OK(rec(ci, kBack, 1));
             HIP_OK(rec(ci, kRefine, 0));
-            HIP_OK(uepm::launch_refine(Hc, n, n, p->X0.as<double2>(), nbr, nb, devals + c0 * nb,
+            HIP_OK(uepm::launch_refine(Hc, n, n, w.X0.as<double2>(), nbr, nb, devals + c0 * nb,
                                        devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
-                                       dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, s,
-                                       p->Y.as<double2>()));
+                                       dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, cs,
+                                       w.Y.as<double2>()));
             HIP_OK(rec(ci, kRefine, 1));
             continue;
         }
         HIP_OK(rec(ci, kTridiag, 0));
         if (p->tri_v1)
-            HIP_OK(uepm::launch_tridiag(Hc, n, n, c, p->d.as<double>(), p->e.as<double>(), p->tau.as<double2>(), s));
+            HIP_OK(uepm::launch_tridiag(Hc, n, n, c, w.d.as<double>(), w.e.as<double>(), w.tau.as<double2>(), cs));
         else {
             // the blocked reduction for the leading steps, the FP64 register grids for the
             // trailing block of size t (the whole matrix when n <= 128); reflectors in place in H
@@ -657,39 +681,43 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             const int s0 = n - t;
             const int64_t nn = (int64_t)n * n;
             if (t < n)
-                HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, p->S.p, Hc, p->d.as<double>(),
-                                                        p->e.as<double>(), p->tau.as<double2>(), s,
+                HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, w.S.p, Hc, w.d.as<double>(),
+                                                        w.e.as<double>(), w.tau.as<double2>(), cs,
                                                         t > 0 ? s0 : n - 1));
             if (t > 0) {
                 double2* B = Hc + s0 + (int64_t)s0 * n;
-                HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, p->d.as<double>() + s0,
-                                                     p->e.as<double>() + s0, p->tau.as<double2>() + s0, n,
-                                                     p->S.as<double2>(), s));
+                HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
+                                                     w.e.as<double>() + s0, w.tau.as<double2>() + s0, n,
+                                                     w.S.as<double2>(), cs));
             }
         }
         HIP_OK(rec(ci, kTridiag, 1));
         HIP_OK(rec(ci, kTriEig, 0));
-        HIP_OK(uepm::launch_tri_eig(p->d.as<double>(), p->e.as<double>(), n, nb, c, devals + c0 * nb,
-                                    need_vec ? p->Z.as<double>() : nullptr, need_vec ? p->W.as<double>() : nullptr,
-                                    need_vec ? kFp64InvSweeps : 0, 0.0, s));
+        HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nb, c, devals + c0 * nb,
+                                    need_vec ? w.Z.as<double>() : nullptr, need_vec ? w.W.as<double>() : nullptr,
+                                    need_vec ? kFp64InvSweeps : 0, 0.0, cs));
         HIP_OK(rec(ci, kTriEig, 1));
         if (need_vec) {
-            double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : p->X.as<double2>();
+            double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X.as<double2>();
             HIP_OK(rec(ci, kBack, 0));
             if (p->bt_v1) {
-                HIP_OK(uepm::launch_backtransform(Hc, n, n, p->tau.as<double2>(), p->d.as<double>(),
-                                                  p->e.as<double>(), devals + c0 * nb, p->Z.as<double>(), Xc,
+                HIP_OK(uepm::launch_backtransform(Hc, n, n, w.tau.as<double2>(), w.d.as<double>(),
+                                                  w.e.as<double>(), devals + c0 * nb, w.Z.as<double>(), Xc,
                                                   dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                  p->G.as<int32_t>(), nb, c, s));
+                                                  p->G.as<int32_t>(), nb, c, cs));
             } else {
-                HIP_OK(uepm::launch_cluster_orth(p->d.as<double>(), p->e.as<double>(), devals + c0 * nb,
-                                                 p->Z.as<double>(), n, nb, c, s));
-                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, p->tau.as<double2>(), p->Z.as<double>(), nb, Xc,
+                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
+                                                 w.Z.as<double>(), n, nb, c, cs));
+                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
                                                 dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                p->G.as<int32_t>(), c, p->bt_bands, s));
+                                                p->G.as<int32_t>(), c, p->bt_bands, cs));
             }
             HIP_OK(rec(ci, kBack, 1));
         }
+    }
+    if (lanes > 1) {
+        HIP_OK(hipEventRecord(p->join, p->side));
+        HIP_OK(hipStreamWaitEvent(s, p->join, 0));
     }
     if (p->timing) {
         HIP_OK(hipStreamSynchronize(s));

@@ -63,6 +63,14 @@ constexpr double kMixedBisectTol = 1e-10;
 constexpr int kFp64InvSweeps = 3;
 constexpr int kMixedInvSweeps = 2;
 
+// Workspace of one chunk in flight. A solve over two or more chunks runs two
+// lanes: even chunks on the caller's stream, odd chunks on the plan's side
+// stream, each with its own workspace.
+struct Lane {
+    DeviceBuffer H, d, e, tau, Z, W, X, S;
+    DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
+};
+
 }  // namespace uepm_host
 
 struct uepm_ctx {
@@ -76,8 +84,10 @@ struct uepm_plan {
     uepm::DevParams P{};
     unsigned flags = 0;
     int N = 0, dim = 0, nb = 0;
-    uepm_host::DeviceBuffer G, H, d, e, tau, Z, W, X, S;
-    uepm_host::DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
+    uepm_host::DeviceBuffer G;
+    uepm_host::Lane lane[2];
+    hipStream_t side = nullptr;  // the second lane's stream (created on first use)
+    hipEvent_t fork = nullptr, join = nullptr;
     uepm::TriPlan tp{}, tp32{};
     uepm_config cfg{};    // requested configuration (0 fields = automatic)
     bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
@@ -91,16 +101,21 @@ struct uepm_plan {
     int64_t stage_launches[uepm_host::kNumStages] = {0, 0, 0, 0, 0};
     ~uepm_plan() {
         for (auto e : ev) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (side) (void)hipStreamDestroy(side);
     }
 };
 
 namespace uepm_host {
 // k-points per launch for a solve of nk points under the plan's configuration
 int64_t plan_chunk(const uepm_plan* p, int64_t nk);
-// workspace of a chunk (bytes) and its allocation; `vectors` reserves the
-// eigenvector scratch used when gradients are asked without eigenvectors
-size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk);
-int plan_reserve(uepm_plan* p, int64_t chunk);
+// lanes a solve of nk points runs (2 when it spans two or more chunks, unless
+// the configuration asks for one)
+int plan_lanes(const uepm_plan* p, int64_t nk);
+// workspace of `lanes` chunks in flight (bytes) and its allocation
+size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk, int lanes = 1);
+int plan_reserve(uepm_plan* p, int64_t chunk, int lanes = 1);
 // applies p->cfg to the derived launch parameters (validated by the caller)
 void plan_apply_config(uepm_plan* p);
 }  // namespace uepm_host
