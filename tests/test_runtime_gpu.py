@@ -72,6 +72,10 @@ def test_two_lanes_equal_one_lane(gpu_ctx, flags):
             assert np.all(np.isfinite(e)) and np.all(np.diff(e, axis=1) >= 0)
         de.free()
         dv.free()
+    p.set_config(max_chunk=256)  # the host-buffer solve: two lanes, results copied back after the join
+    p.set_timing(False)
+    ev, _, _ = p.solve(ks)
+    assert np.array_equal(ev, out["e"])
     stream.close()
     p.close()
 
