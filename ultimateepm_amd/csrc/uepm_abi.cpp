@@ -608,117 +608,123 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     p->stage_name[kBack] = p->bt_v1 ? "backtransform_kernel"
                                     : (mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>");
     p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
-    for (int ci = 0; ci < nchunks; ++ci) {
-        const int64_t c0 = (int64_t)ci * chunk;
-        Lane& w = p->lane[ci % lanes];
-        const hipStream_t cs = ci % lanes ? p->side : s;
-        const int64_t c = std::min(chunk, nk - c0);
-        const double* kc = dk + 3 * c0;
-        double2* Hc = w.H.as<double2>();
-        HIP_OK(rec(ci, kAssemble, 0));
-        HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
-        HIP_OK(rec(ci, kAssemble, 1));
-        if (mixed) {
-            // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
-            // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
-            float2* Vc = w.Vf.as<float2>();
+    // the chunks; on an error the side stream is still joined into the caller's
+    auto run_chunks = [&]() -> int {
+        for (int ci = 0; ci < nchunks; ++ci) {
+            const int64_t c0 = (int64_t)ci * chunk;
+            Lane& w = p->lane[ci % lanes];
+            const hipStream_t cs = ci % lanes ? p->side : s;
+            const int64_t c = std::min(chunk, nk - c0);
+            const double* kc = dk + 3 * c0;
+            double2* Hc = w.H.as<double2>();
+            HIP_OK(rec(ci, kAssemble, 0));
+            HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
+            HIP_OK(rec(ci, kAssemble, 1));
+            if (mixed) {
+                // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
+                // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
+                float2* Vc = w.Vf.as<float2>();
+                HIP_OK(rec(ci, kTridiag, 0));
+                if (p->tri_reg)
+                    HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d.as<double>(), w.e.as<double>(),
+                                                        w.tauf.as<float2>(), w.S32.as<float2>(), cs));
+                else {
+                    // n > 256: the blocked reduction for the leading steps, the register grids
+                    // for the trailing block it leaves in place in Vc
+                    const int t = mixed_c64_tail(p);
+                    const int s0 = n - t;
+                    HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, w.S32.p, Vc, w.d.as<double>(),
+                                                            w.e.as<double>(), w.tauf.as<float2>(), cs,
+                                                            t > 0 ? s0 : n - 1));
+                    if (t > 0) {
+                        const int64_t nn = (int64_t)n * n;
+                        float2* B = Vc + s0 + (int64_t)s0 * n;
+                        HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
+                                                                  w.e.as<double>() + s0, w.tauf.as<float2>() + s0, n,
+                                                                  w.S32.as<float2>(), cs));
+                    }
+                }
+                HIP_OK(rec(ci, kTridiag, 1));
+                HIP_OK(rec(ci, kTriEig, 0));
+                // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
+                // for inverse iteration; the FP64 refinement sets the eigenvalues
+                HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nbr, c, w.Ev.as<double>(),
+                                            w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
+                                            cs));
+                HIP_OK(rec(ci, kTriEig, 1));
+                HIP_OK(rec(ci, kBack, 0));
+                if (p->bt_v1) {
+                    HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, w.tauf.as<float2>(), w.d.as<double>(),
+                                                          w.e.as<double>(), w.Ev.as<double>(), w.Z.as<double>(),
+                                                          w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, cs));
+                } else {
+                    HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
+                                                     w.Z.as<double>(), n, nbr, c, cs));
+                    HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
+                                                    w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
+                                                    cs));
+                }
+                HIP_OK(rec(ci, kBack, 1));
+                HIP_OK(rec(ci, kRefine, 0));
+                HIP_OK(uepm::launch_refine(Hc, n, n, w.X0.as<double2>(), nbr, nb, devals + c0 * nb,
+                                           devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
+                                           dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, cs,
+                                           w.Y.as<double2>()));
+                HIP_OK(rec(ci, kRefine, 1));
+                continue;
+            }
             HIP_OK(rec(ci, kTridiag, 0));
-            if (p->tri_reg)
-                HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d.as<double>(), w.e.as<double>(),
-                                                    w.tauf.as<float2>(), w.S32.as<float2>(), cs));
+            if (p->tri_v1)
+                HIP_OK(uepm::launch_tridiag(Hc, n, n, c, w.d.as<double>(), w.e.as<double>(), w.tau.as<double2>(), cs));
             else {
-                // n > 256: the blocked reduction for the leading steps, the register grids
-                // for the trailing block it leaves in place in Vc
-                const int t = mixed_c64_tail(p);
+                // the blocked reduction for the leading steps, the FP64 register grids for the
+                // trailing block of size t (the whole matrix when n <= 128); reflectors in place in H
+                const int t = uepm::tridiag_reg_c128_tail(n);
                 const int s0 = n - t;
-                HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, w.S32.p, Vc, w.d.as<double>(),
-                                                        w.e.as<double>(), w.tauf.as<float2>(), cs,
-                                                        t > 0 ? s0 : n - 1));
+                const int64_t nn = (int64_t)n * n;
+                if (t < n)
+                    HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, w.S.p, Hc, w.d.as<double>(),
+                                                            w.e.as<double>(), w.tau.as<double2>(), cs,
+                                                            t > 0 ? s0 : n - 1));
                 if (t > 0) {
-                    const int64_t nn = (int64_t)n * n;
-                    float2* B = Vc + s0 + (int64_t)s0 * n;
-                    HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
-                                                              w.e.as<double>() + s0, w.tauf.as<float2>() + s0, n,
-                                                              w.S32.as<float2>(), cs));
+                    double2* B = Hc + s0 + (int64_t)s0 * n;
+                    HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
+                                                         w.e.as<double>() + s0, w.tau.as<double2>() + s0, n,
+                                                         w.S.as<double2>(), cs));
                 }
             }
             HIP_OK(rec(ci, kTridiag, 1));
             HIP_OK(rec(ci, kTriEig, 0));
-            // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
-            // for inverse iteration; the FP64 refinement sets the eigenvalues
-            HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nbr, c, w.Ev.as<double>(),
-                                        w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
-                                        cs));
+            HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nb, c, devals + c0 * nb,
+                                        need_vec ? w.Z.as<double>() : nullptr, need_vec ? w.W.as<double>() : nullptr,
+                                        need_vec ? kFp64InvSweeps : 0, 0.0, cs));
             HIP_OK(rec(ci, kTriEig, 1));
-            HIP_OK(rec(ci, kBack, 0));
-            if (p->bt_v1) {
-                HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, w.tauf.as<float2>(), w.d.as<double>(),
-                                                      w.e.as<double>(), w.Ev.as<double>(), w.Z.as<double>(),
-                                                      w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, cs));
-            } else {
-                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
-                                                 w.Z.as<double>(), n, nbr, c, cs));
-                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
-                                                w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
-                                                cs));
-            }
-            HIP_OK(rec(ci, kBack, 1));
-            HIP_OK(rec(ci, kRefine, 0));
-            HIP_OK(uepm::launch_refine(Hc, n, n, w.X0.as<double2>(), nbr, nb, devals + c0 * nb,
-                                       devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
-                                       dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, cs,
-                                       w.Y.as<double2>()));
-            HIP_OK(rec(ci, kRefine, 1));
-            continue;
-        }
-        HIP_OK(rec(ci, kTridiag, 0));
-        if (p->tri_v1)
-            HIP_OK(uepm::launch_tridiag(Hc, n, n, c, w.d.as<double>(), w.e.as<double>(), w.tau.as<double2>(), cs));
-        else {
-            // the blocked reduction for the leading steps, the FP64 register grids for the
-            // trailing block of size t (the whole matrix when n <= 128); reflectors in place in H
-            const int t = uepm::tridiag_reg_c128_tail(n);
-            const int s0 = n - t;
-            const int64_t nn = (int64_t)n * n;
-            if (t < n)
-                HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, w.S.p, Hc, w.d.as<double>(),
-                                                        w.e.as<double>(), w.tau.as<double2>(), cs,
-                                                        t > 0 ? s0 : n - 1));
-            if (t > 0) {
-                double2* B = Hc + s0 + (int64_t)s0 * n;
-                HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
-                                                     w.e.as<double>() + s0, w.tau.as<double2>() + s0, n,
-                                                     w.S.as<double2>(), cs));
+            if (need_vec) {
+                double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X.as<double2>();
+                HIP_OK(rec(ci, kBack, 0));
+                if (p->bt_v1) {
+                    HIP_OK(uepm::launch_backtransform(Hc, n, n, w.tau.as<double2>(), w.d.as<double>(),
+                                                      w.e.as<double>(), devals + c0 * nb, w.Z.as<double>(), Xc,
+                                                      dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
+                                                      p->G.as<int32_t>(), nb, c, cs));
+                } else {
+                    HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
+                                                     w.Z.as<double>(), n, nb, c, cs));
+                    HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
+                                                    dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
+                                                    p->G.as<int32_t>(), c, p->bt_bands, cs));
+                }
+                HIP_OK(rec(ci, kBack, 1));
             }
         }
-        HIP_OK(rec(ci, kTridiag, 1));
-        HIP_OK(rec(ci, kTriEig, 0));
-        HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nb, c, devals + c0 * nb,
-                                    need_vec ? w.Z.as<double>() : nullptr, need_vec ? w.W.as<double>() : nullptr,
-                                    need_vec ? kFp64InvSweeps : 0, 0.0, cs));
-        HIP_OK(rec(ci, kTriEig, 1));
-        if (need_vec) {
-            double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X.as<double2>();
-            HIP_OK(rec(ci, kBack, 0));
-            if (p->bt_v1) {
-                HIP_OK(uepm::launch_backtransform(Hc, n, n, w.tau.as<double2>(), w.d.as<double>(),
-                                                  w.e.as<double>(), devals + c0 * nb, w.Z.as<double>(), Xc,
-                                                  dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                  p->G.as<int32_t>(), nb, c, cs));
-            } else {
-                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
-                                                 w.Z.as<double>(), n, nb, c, cs));
-                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
-                                                dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                p->G.as<int32_t>(), c, p->bt_bands, cs));
-            }
-            HIP_OK(rec(ci, kBack, 1));
-        }
-    }
+        return UEPM_OK;
+    };
+    const int rc = run_chunks();
     if (lanes > 1) {
         HIP_OK(hipEventRecord(p->join, p->side));
         HIP_OK(hipStreamWaitEvent(s, p->join, 0));
     }
+    if (rc != UEPM_OK) return rc;
     if (p->timing) {
         HIP_OK(hipStreamSynchronize(s));
         for (int st = 0; st < kNumStages; ++st) {
