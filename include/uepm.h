@@ -133,7 +133,9 @@ int uepm_solve(uepm_plan* plan, const double* k, int64_t nk, double* evals, doub
                double* grads);
 
 /* Same on device buffers, enqueued on `stream` (a hipStream_t; NULL = the
- * default stream).  Returns after enqueueing. */
+ * default stream).  Returns after enqueueing.  A solve of more than one chunk
+ * (uepm_config.lanes) also runs on a stream the plan owns, forked from and
+ * joined back into `stream`: later work on `stream` sees every result. */
 int uepm_solve_device(uepm_plan* plan, const double* d_k, int64_t nk, double* d_evals,
                       double* d_evecs, double* d_grads, void* stream);
 
