@@ -36,6 +36,14 @@
  *                         bessel_func.hpp compiled in oracle/_ref, scipy spherical_jn
  *   i_2                   scipy spherical_in(2, x); bit-equal to oracle/_ref
  *   F_l, gaussian F_2     numerical quadrature of the square / gaussian well overlap
+ *   non-local element     the partial-wave definition (4 pi / Omega) sum_l (2l+1) A_l P_l
+ *                         int w_l j_l j_l r^2 dr by quadrature, summed over the two sites
+ *                         (tests/test_oracle_compositions.py), with mutants of this file
+ *                         (prefactors, beta term, P_2, radii, sign of the sin term) failing it
+ *   spin-orbit block      the per-site sum of -i lambda_s (K x K').sigma e^{+-i th} with
+ *                         Slater radial factors by quadrature; mutants (lambda_s/lambda_a
+ *                         swapped, sin sign, alpha on the wrong species, periods) fail it
+ *   assembled H(k)        every element of SetMatrix's matrix from those definitions
  *   local V               bit-equal to Pseudopotential::GetValue compiled in oracle/_ref
  *   k path                bit-equal to SymmetryPoints::GeneratePoints in oracle/_ref
  *   basis                 closed-form N table and brute-force enumeration
