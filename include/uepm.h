@@ -128,16 +128,35 @@ int uepm_plan_dim(const uepm_plan* plan);
  *            band's vector contiguous and of unit norm (phase arbitrary, as
  *            Eigen's); NULL unless the plan has UEPM_VECTORS
  *   grads  : nk x nb x 3 doubles in eV*m; NULL unless UEPM_GRADIENTS
- * Host buffers; the call is synchronous. */
+ * Host buffers; the call is synchronous.  A non-finite k-point ->
+ * UEPM_ERR_INVALID_ARG before any device work.  A k-point whose eigen-solve
+ * fails (H(k) not finite, e.g. overflow at huge |k|) gets NaN eigenvalues and
+ * vectors; the others are solved and every output is written, and the call
+ * returns UEPM_ERR_NOT_CONVERGED (the reference's Eigen solver reports
+ * NoConvergence and Diagonalize throws "Eigenvalue decomposition failed",
+ * Hamiltonian.cpp:130-133). */
 int uepm_solve(uepm_plan* plan, const double* k, int64_t nk, double* evals, double* evecs,
                double* grads);
 
 /* Same on device buffers, enqueued on `stream` (a hipStream_t; NULL = the
  * default stream).  Returns after enqueueing.  A solve of more than one chunk
  * (uepm_config.lanes) also runs on a stream the plan owns, forked from and
- * joined back into `stream`: later work on `stream` sees every result. */
+ * joined back into `stream`: later work on `stream` sees every result.
+ * Solves of one plan are ordered whatever streams they are enqueued on: each
+ * waits for the plan's previous solve (outside graph capture), since they
+ * share the plan's workspace.  k-points are not checked on the host; failed
+ * k-points (NaN eigenvalues and vectors) are counted on the device and read
+ * with uepm_plan_status. */
 int uepm_solve_device(uepm_plan* plan, const double* d_k, int64_t nk, double* d_evals,
                       double* d_evecs, double* d_grads, void* stream);
+
+/* Failures of the plan's last solve: waits for `stream` (the stream the solve
+ * was enqueued on), then writes the number of k-points whose eigen-solve
+ * failed and the lowest index among them (-1 when none); either pointer may
+ * be NULL.  Returns UEPM_ERR_NOT_CONVERGED when any failed, else UEPM_OK.
+ * Replaces the per-k ComputationInfo check of Hamiltonian::Diagonalize
+ * (Hamiltonian.cpp:130-133). */
+int uepm_plan_status(uepm_plan* plan, void* stream, int64_t* n_failed, int64_t* first_failed);
 
 /* Hamiltonian::SetMatrix + the hermitisation step of Diagonalize
  * (Hamiltonian.cpp:50-127): writes the full dim x dim complex128 matrix
@@ -147,10 +166,13 @@ int uepm_assemble_device(uepm_plan* plan, const double* d_k, int64_t nk, double*
 
 /* Per-stage device timing of the last uepm_solve_device call, enabled with
  * uepm_plan_set_timing(plan, 1): for stage index `stage` (0-based; beyond
- * the last -> UEPM_ERR_OUT_OF_RANGE) the summed kernel time in milliseconds
- * (hipEvents recorded on the launch stream), the number of launches and the
- * kernel's name.  No counterpart in the reference (it times the whole k loop
- * in seconds, BandStructure.cpp:153,198-199). */
+ * the last -> UEPM_ERR_OUT_OF_RANGE) the kernel time in milliseconds, the
+ * number of launches and the kernel's name.  The time is the sum over chunks
+ * of the interval between hipEvents recorded around the stage on the stream
+ * its chunk ran on; with two lanes those intervals overlap in time, so the
+ * stage sums add up to about twice the wall time.  In timing mode the solve
+ * synchronises `stream` before it returns.  No counterpart in the reference
+ * (it times the whole k loop in seconds, BandStructure.cpp:153,198-199). */
 int uepm_plan_set_timing(uepm_plan* plan, int enable);
 int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t* launches,
                           const char** name);
@@ -162,7 +184,7 @@ typedef struct uepm_config {
                           leading steps, then the register grids); 1 LDS-blocked
                           reduction only; 2 register-resident complex64 reduction
                           (UEPM_MIXED plans, dim <= 256)                              */
-    int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = auto) */
+    int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = 512)  */
     int32_t bt_bands;  /* bands per back-transform workgroup: 4, 8 or 16 (0 = 16)    */
     int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */
     int32_t lanes;     /* chunks in flight: 0 automatic (2 when a solve spans two or
@@ -170,7 +192,8 @@ typedef struct uepm_config {
                           a plan-owned stream, twice the workspace); 1 one chunk    */
 } uepm_config;
 int uepm_plan_get_config(const uepm_plan* plan, uepm_config* cfg);
-/* Values outside the ranges above -> UEPM_ERR_INVALID_ARG (plan unchanged). */
+/* Values outside the ranges above, or a plan held by a batch graph ->
+ * UEPM_ERR_INVALID_ARG (plan unchanged). */
 int uepm_plan_set_config(uepm_plan* plan, const uepm_config* cfg);
 /* Autotuner: times every candidate configuration on the device k-points
  * d_k[0..nk) (one warm-up and two timed solves each, on `stream`), keeps the
@@ -187,7 +210,12 @@ int uepm_plan_reserve(uepm_plan* plan, int64_t nk);
  * once into one hipGraph and replayed with one launch.  Items of one plan run
  * in order on one branch; plans are spread over `branches` concurrent branches,
  * heaviest first.  Every plan's workspace is reserved at creation, so the graph
- * allocates nothing.  Plans must not be used elsewhere while the graph runs.
+ * allocates nothing.  The graph addresses the plans' workspaces, so until
+ * uepm_batch_destroy each plan is held: uepm_plan_destroy, _set_config and
+ * _tune return UEPM_ERR_INVALID_ARG, and so does a solve of the plan that
+ * would need a larger workspace than the one reserved.  Other solves of a held
+ * plan reuse that workspace: they must not run while the graph runs.  Each
+ * replay clears and fills the plans' status words (uepm_plan_status).
  * No counterpart in the reference (its k loop is per material). */
 typedef struct uepm_batch_item {
     uepm_plan* plan;
@@ -210,6 +238,9 @@ int uepm_batch_destroy(uepm_batch* batch);
  * per (material, basis, flags, bands) with resident workspaces; idle plans are
  * evicted oldest first when the budget needs room.  A request that cannot fit
  * even alone fails with UEPM_ERR_NO_MEMORY.  Eigenvalues only (no vectors).
+ * A request with a non-finite k-point is refused at submission
+ * (UEPM_ERR_INVALID_ARG); one whose eigen-solve fails for some k-point ends
+ * with UEPM_ERR_NOT_CONVERGED, its other eigenvalues returned.
  * No counterpart in the reference. */
 typedef struct uepm_request {
     const uepm_material* material;

@@ -15,6 +15,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -102,6 +103,26 @@ static int run_cpu(int argc, char** argv) {
     b2.Initialize(ms.materials.at("Ge"), 8, std::vector<Vector3D<double>>{Vector3D<double>(0.1, 0.2, 0.3)}, 10, true, true);
     CHECK(b2.get_number_of_bands() == 16 && b2.get_basis_vectors().size() == 137);
     CHECK(throws<std::runtime_error>([&] { BandStructure b3; b3.Initialize(ms.materials.at("Si"), 8, path, 100, 1, false, false); }));
+    // a non-finite k-point: "Eigenvalue decomposition failed" (Hamiltonian.cpp:130-133), before any device work
+    {
+        const double nan = std::nan("");
+        const std::vector<Vector3D<int>> G = bs.get_basis_vectors();
+        Hamiltonian h(ms.materials.at("Si"), G);
+        h.SetMatrix(Vector3D<double>(0.1, nan, 0.2), true, false);
+        bool msg_ok = false;
+        try {
+            h.Diagonalize(false);
+        } catch (const std::runtime_error& e) {
+            msg_ok = std::string(e.what()) == "Eigenvalue decomposition failed";
+        }
+        CHECK(msg_ok);
+        BandStructure b4;
+        b4.Initialize(ms.materials.at("Si"), 8,
+                      std::vector<Vector3D<double>>{Vector3D<double>(0.0, 0.0, 0.0),
+                                                    Vector3D<double>(std::numeric_limits<double>::infinity(), 0, 0)},
+                      10, false, false);
+        CHECK(throws<std::runtime_error>([&] { b4.Compute(); }));
+    }
 
     Vector3D<double> a(1, 2, 3), b(-2, 0.5, 4);
     CHECK(a * b == -2 + 1 + 12);

@@ -173,9 +173,15 @@ class Plan:
         ev = np.zeros((nk, self.nb))
         vec = np.zeros((nk, self.nb, self.dim), dtype=np.complex128) if vectors else None
         gr = np.zeros((nk, self.nb, 3)) if gradients else None
-        check(lib().uepm_solve(self._h, _dp(k), nk, _dp(ev),
-                               vec.ctypes.data_as(C.POINTER(C.c_double)) if vectors else None,
-                               _dp(gr) if gradients else None))
+        rc = lib().uepm_solve(self._h, _dp(k), nk, _dp(ev),
+                              vec.ctypes.data_as(C.POINTER(C.c_double)) if vectors else None,
+                              _dp(gr) if gradients else None)
+        if rc == _abi.UEPM_ERR_NOT_CONVERGED:
+            # the other k-points are solved: the error carries every output
+            err = _abi.UepmError(rc, lib().uepm_last_error().decode())
+            err.result = (ev, vec, gr)
+            raise err
+        check(rc)
         return ev, vec, gr
 
     def solve_device(self, k_ptr: int, nk: int, evals_ptr: int, evecs_ptr: int = 0, grads_ptr: int = 0,
@@ -183,6 +189,14 @@ class Plan:
         check(lib().uepm_solve_device(self._h, C.c_void_p(k_ptr), nk, C.c_void_p(evals_ptr),
                                       C.c_void_p(evecs_ptr or None), C.c_void_p(grads_ptr or None),
                                       C.c_void_p(stream or None)))
+
+    def status(self, stream: int = 0):
+        """(failed k-points, first failed index or -1) of the last solve, after waiting for `stream`."""
+        n, first = C.c_int64(0), C.c_int64(0)
+        rc = lib().uepm_plan_status(self._h, C.c_void_p(stream or None), C.byref(n), C.byref(first))
+        if rc not in (_abi.UEPM_OK, _abi.UEPM_ERR_NOT_CONVERGED):
+            check(rc)
+        return n.value, first.value
 
     def assemble_device(self, k_ptr: int, nk: int, H_ptr: int, stream: int = 0):
         check(lib().uepm_assemble_device(self._h, C.c_void_p(k_ptr), nk, C.c_void_p(H_ptr), C.c_void_p(stream or None)))
@@ -283,7 +297,12 @@ class Server:
         nk, nb = self._pending.pop(ticket)
         ev = np.zeros((nk, nb))
         lat, que = C.c_double(0), C.c_double(0)
-        check(lib().uepm_server_wait(self._h, ticket, _dp(ev), C.byref(lat), C.byref(que)))
+        rc = lib().uepm_server_wait(self._h, ticket, _dp(ev), C.byref(lat), C.byref(que))
+        if rc == _abi.UEPM_ERR_NOT_CONVERGED:
+            err = _abi.UepmError(rc, lib().uepm_last_error().decode())
+            err.result = (ev, lat.value, que.value)
+            raise err
+        check(rc)
         return ev, lat.value, que.value
 
     def stats(self) -> dict:

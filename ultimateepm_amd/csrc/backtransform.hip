@@ -125,7 +125,14 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
                                                       const VT* __restrict__ tauall, const double* __restrict__ Z,
                                                       int nb, double2* __restrict__ X, double* __restrict__ grads,
                                                       DevParams P, const double* __restrict__ kpts,
-                                                      const int32_t* __restrict__ G) {
+                                                      const int32_t* __restrict__ G, const int* __restrict__ kstat,
+                                                      unsigned long long* __restrict__ status, int64_t k0) {
+    // the last stage of the FP64 form: a k-point the tridiagonal stage flagged
+    // counts as failed (its eigenvalues and vectors are NaN)
+    if (status && blockIdx.y == 0 && threadIdx.x == 0 && kstat[blockIdx.x]) {
+        atomicAdd(&status[0], 1ull);
+        atomicMin(&status[1], (unsigned long long)(k0 + blockIdx.x));
+    }
     // register budget: prefetch the next reflector when two copies of v fit
     // beside the bands, keep one copy when it fits, else re-read v (an L1 hit)
     constexpr int CW = 2 * (int)sizeof(R) / 4;  // VGPRs per complex
@@ -259,14 +266,8 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
 namespace {
 // Bands per 16-lane row of apply_q_kernel (1, 2 or 4): as many as keep the
 // kernel within its registers at two or more waves per SIMD -- complex64 4 up
-// to 192 rows and 2 up to 256, complex128 2 up to 192 -- or UEPM_BT_BPL.
+// to 192 rows and 2 up to 256, complex128 2 up to 192.
 int apply_bands_per_lane(bool f32, int n) {
-    static const int forced = [] {
-        const char* e = std::getenv("UEPM_BT_BPL");
-        const int b = e ? std::atoi(e) : 0;
-        return b >= 4 ? 4 : b == 2 ? 2 : b == 1 ? 1 : 0;
-    }();
-    if (forced) return forced;
     if (f32) return n <= 192 ? 4 : n <= 256 ? 2 : 1;
     return n <= 192 ? 2 : 1;
 }
@@ -274,7 +275,8 @@ int apply_bands_per_lane(bool f32, int n) {
 template <typename R, typename VT>
 hipError_t launch_apply(const VT* dV, int n, int ld, const VT* dtau, const double* dZ, int nb, double2* dX,
                         double* dgrads, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                        int bands_per_group, hipStream_t s) {
+                        const int* kstat, unsigned long long* status, int64_t k0, int bands_per_group,
+                        hipStream_t s) {
     // bands_per_group (4, 8 or 16) bands per workgroup, BPL of them per 16-lane row
     const int cap = (bands_per_group == 4 || bands_per_group == 8) ? bands_per_group : 16;
     const int bpw = nb >= cap ? cap : ((nb + 3) / 4) * 4;
@@ -285,7 +287,7 @@ hipError_t launch_apply(const VT* dV, int n, int ld, const VT* dtau, const doubl
     const dim3 grid((unsigned)nk, (unsigned)((nb + bpw - 1) / bpw)), block(16 * (bpw / bpl));
 #define UEPM_APPLY(ROWS, BPL)                                                                                       \
     hipLaunchKernelGGL((apply_q_kernel<R, VT, ROWS, BPL>), grid, block, 0, s, dV, n, ld, dtau, dZ, nb, dX, dgrads, \
-                       P, dk, dG)
+                       P, dk, dG, kstat, status, k0)
 #define UEPM_APPLY_ROWS(BPL)                    \
     if (n <= 64) UEPM_APPLY(4, BPL);            \
     else if (n <= 128) UEPM_APPLY(8, BPL);      \
@@ -315,14 +317,17 @@ hipError_t launch_cluster_orth(const double* dd, const double* de, const double*
 
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                              int64_t nk, int bands_per_group, hipStream_t s) {
-    return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, bands_per_group, s);
+                              int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
+                              int bands_per_group, hipStream_t s) {
+    return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, kstat, status, k0,
+                                         bands_per_group, s);
 }
 
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
                               int bands_per_group, hipStream_t s) {
-    return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, bands_per_group, s);
+    return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, nullptr, nullptr, 0,
+                                       bands_per_group, s);
 }
 
 }  // namespace uepm

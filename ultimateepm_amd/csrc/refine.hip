@@ -41,15 +41,6 @@ __device__ __forceinline__ void wsync() {
 }
 
 constexpr int kRefBS = 256;
-
-// UEPM_REFINE_V1=1 keeps nbr > 16 on the VALU refine_kernel (A/B reference).
-bool refine_mfma2_enabled() {
-    static const bool on = [] {
-        const char* v = std::getenv("UEPM_REFINE_V1");
-        return !(v && v[0] == '1');
-    }();
-    return on;
-}
 constexpr int kNbrMax = 28;
 constexpr int kChunk = 64;
 // column quads loaded ahead of their MFMAs in Y = H X (one / two band tiles)
@@ -94,8 +85,10 @@ __device__ __forceinline__ void upper_solve_h(int nbr, const double2* __restrict
 // Cholesky S = L L^H, M' = L^{-1} M L^{-H}, parallel-ordered Jacobi M' = Z Theta Z^H,
 // C = L^{-H} Z (in Wt), Ritz values th and the ascending order perm.
 // Inputs Ss = S, Mm = M (row-major, stride LDM); Ss, Mm, Zm, Wt are overwritten.
+// Returns false (the same in every lane) when the Jacobi sweeps end at their
+// cap with a pair still above the rotation test.
 template <int LDM>
-__device__ void small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* __restrict__ Mm,
+__device__ bool small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* __restrict__ Mm,
                                     double2* __restrict__ Zm, double2* __restrict__ Wt, double* __restrict__ th,
                                     int* __restrict__ perm, double4* __restrict__ rot, int lane) {
     const int i = lane;
@@ -197,17 +190,23 @@ __device__ void small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* 
         const double scale = fabs(a) + fabs(b) + 1e-300;
         return ac <= 1e-15 * scale || ac * ac <= 1e-13 * scale * fabs(b - a);
     };
+    // every pair below the rotation test (wave-uniform)
+    auto all_negligible = [&]() {
+        int big = 0;
+        for (int pq = i; pq < LDM * LDM; pq += 64) {
+            const int p = pq / LDM, q = pq % LDM;
+            if (p < q && q < nbr && !negligible(p, q)) big = 1;
+        }
+        for (int o = 32; o > 0; o >>= 1) big |= __shfl_xor(big, o, 64);
+        return big == 0;
+    };
+    bool converged = false;
     for (int sweep = 0; sweep < 20; ++sweep) {
         // a sweep that finds every pair negligible at its start rotates nothing
         // (no round changes M): test all pairs at once and stop
-        {
-            int big = 0;
-            for (int pq = i; pq < LDM * LDM; pq += 64) {
-                const int p = pq / LDM, q = pq % LDM;
-                if (p < q && q < nbr && !negligible(p, q)) big = 1;
-            }
-            for (int o = 32; o > 0; o >>= 1) big |= __shfl_xor(big, o, 64);
-            if (!big) break;
+        if (all_negligible()) {
+            converged = true;
+            break;
         }
         int rotated = 0;
         for (int round = 0; round < m - 1; ++round) {
@@ -288,8 +287,12 @@ __device__ void small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* 
                 wsync();
             }
         }
-        if (!rotated) break;
+        if (!rotated) {
+            converged = true;
+            break;
+        }
     }
+    if (!converged) converged = all_negligible();
     wsync();
     // C = L^{-H} Z (back substitution with L^H, lane = column of Z)
     if constexpr (LDM <= 16) {
@@ -327,6 +330,23 @@ __device__ void small_rayleigh_ritz(int nbr, double2* __restrict__ Ss, double2* 
         perm[rank] = i;
     }
     wsync();
+    return converged;
+}
+
+// Failure of the k-point's eigen-solve (one wave, after the small problem):
+// the tridiagonal stage's flag, Jacobi sweeps that did not converge, or a
+// non-finite Ritz value among the nb returned.  Failures add to status[0] and
+// lower status[1] to the k-point's index in the whole solve.
+__device__ void report_ritz_status(const int* kstat, unsigned long long* status, int64_t k0, int64_t kk, int nb,
+                                   const double* th, const int* perm, bool converged, int lane) {
+    if (!status) return;
+    int bad = 0;
+    for (int b = lane; b < nb; b += 64) bad |= isfinite(th[perm[b]]) ? 0 : 1;
+    for (int o = 32; o > 0; o >>= 1) bad |= __shfl_xor(bad, o, 64);
+    if (lane == 0 && (bad || !converged || (kstat && kstat[kk]))) {
+        atomicAdd(&status[0], 1ull);
+        atomicMin(&status[1], (unsigned long long)(k0 + kk));
+    }
 }
 
 // X1 = X0 C[:, perm[b]] for the lowest nb Ritz pairs: eigenvalues, unit
@@ -407,92 +427,6 @@ __device__ void write_ritz(const double2* __restrict__ X, int n, int nbr, int nb
     }
 }
 
-template <int BS>
-__global__ __launch_bounds__(BS) void refine_kernel(const double2* __restrict__ H, int n, int ld,
-                                                    double2* __restrict__ Xall, double2* __restrict__ Yall, int nbr,
-                                                    int nb, double* __restrict__ evals, double2* __restrict__ evecs,
-                                                    double* __restrict__ grads, DevParams P,
-                                                    const double* __restrict__ kpts, const int32_t* __restrict__ G) {
-    __shared__ double2 Xc[kNbrMax * kChunk];        // chunk of X0 rows for Y = H X0
-    __shared__ double2 Mm[kNbrMax * kNbrMax];       // M, then M'
-    __shared__ double2 Ss[kNbrMax * kNbrMax];       // S, then L
-    __shared__ double2 Zm[kNbrMax * kNbrMax];       // Jacobi vectors, then C = L^{-H} Z
-    __shared__ double2 Wt[kNbrMax * kNbrMax];
-    __shared__ double th[kNbrMax];
-    __shared__ int perm[kNbrMax];
-    __shared__ double4 rot[kNbrMax / 2 + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t kk = blockIdx.x;
-    const double2* A = H + kk * (int64_t)ld * n;
-    double2* X = Xall + kk * (int64_t)nbr * n;  // X[b * n + r]
-    double2* Y = Yall + kk * (int64_t)nbr * n;
-
-    // ---- Y = H X0 (H column-major full, Hermitian)
-    for (int r0 = 0; r0 < n; r0 += BS) {
-        const int r = r0 + tid;
-        double2 acc[kNbrMax];
-#pragma unroll
-        for (int b = 0; b < kNbrMax; ++b) acc[b] = make_double2(0.0, 0.0);
-        for (int c0 = 0; c0 < n; c0 += kChunk) {
-            const int cn = min(kChunk, n - c0);
-            __syncthreads();
-            for (int i = tid; i < nbr * kChunk; i += BS) {
-                const int b = i / kChunk, c = i % kChunk;
-                Xc[b * kChunk + c] = c < cn ? X[b * n + c0 + c] : make_double2(0.0, 0.0);
-            }
-            __syncthreads();
-            if (r < n) {
-                for (int c = 0; c < cn; ++c) {
-                    const double2 h = A[r + (int64_t)(c0 + c) * ld];
-#pragma unroll
-                    for (int b = 0; b < kNbrMax; ++b)
-                        if (b < nbr) {
-                            const double2 x = Xc[b * kChunk + c];
-                            acc[b].x = fma(h.x, x.x, fma(-h.y, x.y, acc[b].x));
-                            acc[b].y = fma(h.x, x.y, fma(h.y, x.x, acc[b].y));
-                        }
-                }
-            }
-        }
-        if (r < n)
-#pragma unroll
-            for (int b = 0; b < kNbrMax; ++b)
-                if (b < nbr) Y[b * n + r] = acc[b];
-    }
-    __syncthreads();
-    __threadfence_block();
-    // ---- S = X0^H X0, M = X0^H Y (one wave per (a, b) pair, lanes over rows)
-    for (int pr = wv; pr < nbr * nbr; pr += BS / 64) {
-        const int a = pr / nbr, b = pr % nbr;
-        double sx = 0, sy = 0, mx = 0, my = 0;
-        for (int r = lane; r < n; r += 64) {
-            const double2 xa = X[a * n + r];
-            const double2 s = zmulc(xa, X[b * n + r]);
-            const double2 m = zmulc(xa, Y[b * n + r]);
-            sx += s.x;
-            sy += s.y;
-            mx += m.x;
-            my += m.y;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            sx += __shfl_xor(sx, o, 64);
-            sy += __shfl_xor(sy, o, 64);
-            mx += __shfl_xor(mx, o, 64);
-            my += __shfl_xor(my, o, 64);
-        }
-        if (lane == 0) {
-            Ss[a * kNbrMax + b] = make_double2(sx, sy);
-            Mm[a * kNbrMax + b] = make_double2(mx, my);
-        }
-    }
-    __syncthreads();
-    // ---- small dense problem in wave 0
-    if (wv == 0) small_rayleigh_ritz<kNbrMax>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
-    __syncthreads();
-    write_ritz<BS, kNbrMax>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
-                            reinterpret_cast<double*>(Mm));
-}
-
 // ---------------------------------------------------------------------------
 // nbr <= 16: Y = H X0 on FP64 MFMA (16x16x4), one 16-row tile of Y at a time;
 // the accumulator layout of a Y tile (lane l, item v: row 4v + l/16, band l%16)
@@ -511,7 +445,8 @@ __global__ __launch_bounds__(BS, 4) void refine_mfma_kernel(const double2* __res
                                                          double* __restrict__ evals, double2* __restrict__ evecs,
                                                          double* __restrict__ grads, DevParams P,
                                                          const double* __restrict__ kpts,
-                                                         const int32_t* __restrict__ G) {
+                                                         const int32_t* __restrict__ G, const int* __restrict__ kstat,
+                                                         unsigned long long* __restrict__ status, int64_t k0) {
     constexpr int NW = BS / 64;
     // per-wave partial S_re, S_im, M_re, M_im; once S and M are summed (in
     // registers) the same LDS holds the small problem's M, S, Z and W, so a
@@ -607,7 +542,10 @@ __global__ __launch_bounds__(BS, 4) void refine_mfma_kernel(const double2* __res
         }
     }
     __syncthreads();
-    if (wv == 0) small_rayleigh_ritz<16>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+    if (wv == 0) {
+        const bool conv = small_rayleigh_ritz<16>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+        report_ritz_status(kstat, status, k0, kk, nb, th, perm, conv, lane);
+    }
     __syncthreads();
     write_ritz<BS, 16>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
                        reinterpret_cast<double*>(Mm));
@@ -624,7 +562,8 @@ __global__ __launch_bounds__(BS, 2) void refine_mfma2_kernel(const double2* __re
                                                           double* __restrict__ evals, double2* __restrict__ evecs,
                                                           double* __restrict__ grads, DevParams P,
                                                           const double* __restrict__ kpts,
-                                                          const int32_t* __restrict__ G) {
+                                                          const int32_t* __restrict__ G, const int* __restrict__ kstat,
+                                                          unsigned long long* __restrict__ status, int64_t k0) {
     constexpr int NW = BS / 64;
     // One partial slot filled by the waves in turn; once S and M are formed the
     // slot is reused for the Jacobi work arrays Zm and Wt (66 KB of LDS in all,
@@ -742,29 +681,27 @@ __global__ __launch_bounds__(BS, 2) void refine_mfma2_kernel(const double2* __re
         }
     }
     __syncthreads();  // part is free from here: Zm and Wt take its place
-    if (wv == 0) small_rayleigh_ritz<32>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+    if (wv == 0) {
+        const bool conv = small_rayleigh_ritz<32>(nbr, Ss, Mm, Zm, Wt, th, perm, rot, lane);
+        report_ritz_status(kstat, status, k0, kk, nb, th, perm, conv, lane);
+    }
     __syncthreads();
     write_ritz<BS, 32>(X, n, nbr, nb, Wt, th, perm, kk, evals, evecs, grads, P, kpts, G,
                        reinterpret_cast<double*>(Mm));
 }
 
-const char* refine_kernel_name(int nbr) {
-    return nbr <= 16 ? "refine_mfma_kernel" : refine_mfma2_enabled() ? "refine_mfma2_kernel" : "refine_kernel";
-}
+const char* refine_kernel_name(int nbr) { return nbr <= 16 ? "refine_mfma_kernel" : "refine_mfma2_kernel"; }
 
 hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr, int nb, double* devals,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                         int64_t nk, hipStream_t s, double2* dY) {
+                         int64_t nk, const int* kstat, unsigned long long* status, int64_t k0, hipStream_t s) {
     if (nbr > kNbrMax || nb > nbr) return hipErrorInvalidValue;
     if (nbr <= 16)
         hipLaunchKernelGGL(refine_mfma_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
-                           nb, devals, devecs, dgrads, P, dk, dG);
-    else if (refine_mfma2_enabled())
-        hipLaunchKernelGGL(refine_mfma2_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
-                           nb, devals, devecs, dgrads, P, dk, dG);
+                           nb, devals, devecs, dgrads, P, dk, dG, kstat, status, k0);
     else
-        hipLaunchKernelGGL(refine_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, dY, nbr,
-                           nb, devals, devecs, dgrads, P, dk, dG);
+        hipLaunchKernelGGL(refine_mfma2_kernel<kRefBS>, dim3((unsigned)nk), dim3(kRefBS), 0, s, dH, n, ld, dX, nbr,
+                           nb, devals, devecs, dgrads, P, dk, dG, kstat, status, k0);
     return hipGetLastError();
 }
 

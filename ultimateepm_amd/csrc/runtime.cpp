@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -40,11 +41,15 @@ struct uepm_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;
+    // plans whose lane-0 workspace the graph's kernels address: pinned for the
+    // graph's life (their buffers are not reallocated, the plans not destroyed)
+    std::vector<uepm_plan*> pinned;
     ~uepm_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (auto e : events) (void)hipEventDestroy(e);
         for (auto s : streams) (void)hipStreamDestroy(s);
+        for (auto* p : pinned) --p->pins;
     }
 };
 
@@ -146,6 +151,10 @@ int uepm_batch_create(uepm_ctx* ctx, const uepm_batch_item* items, int nitems, i
     const hipError_t ei = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
     if (ei != hipSuccess)
         return cleanup(fail(UEPM_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei)));
+    for (auto& j : jobs) {
+        ++j.first->pins;
+        B->pinned.push_back(j.first);
+    }
     *out = B;
     return UEPM_OK;
 }
@@ -205,6 +214,7 @@ struct Slot {
     uepm_host::DeviceBuffer dk, dev;
     double* hk = nullptr;   // pinned staging
     double* hev = nullptr;
+    unsigned long long* hstat = nullptr;  // the plan's status words after the solve (pinned)
     size_t hk_cap = 0, hev_cap = 0;
     std::shared_ptr<Request> req;
     PooledPlan* plan = nullptr;
@@ -312,14 +322,20 @@ PooledPlan* acquire_plan(uepm_server* S, const Request& r, const PlanKey& key, i
     return pp;
 }
 
-void finish(uepm_server* S, Slot& sl, int status, const std::string& err) {
+void finish(uepm_server* S, Slot& sl, int status, const std::string& err, bool solved = false) {
     auto r = sl.req;
     r->t_done = Clock::now();
-    if (status == UEPM_OK) r->evals.assign(sl.hev, sl.hev + (size_t)r->nk * r->nb);
+    std::string msg = err;
+    if (solved && sl.hstat[0] != 0) {  // the eigenvalues of the other k-points are still returned
+        status = UEPM_ERR_NOT_CONVERGED;
+        msg = "Eigenvalue decomposition failed for " + std::to_string(sl.hstat[0]) +
+              " k-point(s), the first at index " + std::to_string(sl.hstat[1]);
+    }
+    if (status == UEPM_OK || solved) r->evals.assign(sl.hev, sl.hev + (size_t)r->nk * r->nb);
     {
         std::lock_guard<std::mutex> g(S->mu);
         r->status = status;
-        r->error = err;
+        r->error = msg;
         ++S->stats.completed;
         S->stats.kpoints += (uint64_t)r->nk;
     }
@@ -341,7 +357,7 @@ void worker_main(uepm_server* S) {
             if (!sl.req) continue;
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipSuccess) {
-                finish(S, sl, UEPM_OK, "");
+                finish(S, sl, UEPM_OK, "", true);
                 progress = true;
             } else if (q != hipErrorNotReady) {
                 finish(S, sl, UEPM_ERR_DEVICE, std::string("request failed on the device: ") + hipGetErrorString(q));
@@ -403,6 +419,8 @@ void worker_main(uepm_server* S) {
             if (st == UEPM_OK &&
                 (hipMemcpyAsync(sl.hev, sl.dev.p, ecount * sizeof(double), hipMemcpyDeviceToHost, sl.stream) !=
                      hipSuccess ||
+                 hipMemcpyAsync(sl.hstat, pp->plan->status.p, 2 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, sl.stream) != hipSuccess ||
                  hipEventRecord(sl.done, sl.stream) != hipSuccess))
                 st = fail(UEPM_ERR_DEVICE, "server: copy out");
             if (st != UEPM_OK) {
@@ -436,7 +454,8 @@ int uepm_server_create(uepm_ctx* ctx, uint64_t budget, int max_inflight, uepm_se
     S->slots.resize(max_inflight);
     for (auto& sl : S->slots) {
         if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+            hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void**)&sl.hstat, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
             delete S;
             return fail(UEPM_ERR_DEVICE, "uepm_server_create: stream or event creation failed");
         }
@@ -451,6 +470,9 @@ int uepm_server_submit(uepm_server* S, const uepm_request* q, uint64_t* ticket) 
         return fail(UEPM_ERR_INVALID_ARG, "uepm_server_submit: null argument");
     if (q->flags & (UEPM_VECTORS | UEPM_GRADIENTS))
         return fail(UEPM_ERR_INVALID_ARG, "uepm_server_submit: requests return eigenvalues only");
+    for (int64_t i = 0; i < 3 * q->nk; ++i)
+        if (!std::isfinite(q->k[i]))
+            return fail(UEPM_ERR_INVALID_ARG, "uepm_server_submit: k-point " + std::to_string(i / 3) + " is not finite");
     auto r = std::make_shared<Request>();
     r->mat = *q->material;
     r->shells = q->nearest_neighbors;
@@ -491,8 +513,9 @@ int uepm_server_wait(uepm_server* S, uint64_t ticket, double* evals, double* lat
     }
     if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
     if (queue_s) *queue_s = std::chrono::duration<double>(r->t_admit - r->t_submit).count();
-    if (r->status != UEPM_OK) return fail(r->status, r->error);
+    // a k-point whose solve failed leaves NaN eigenvalues; the others are returned
     if (evals && !r->evals.empty()) std::memcpy(evals, r->evals.data(), r->evals.size() * sizeof(double));
+    if (r->status != UEPM_OK) return fail(r->status, r->error);
     return UEPM_OK;
 }
 
@@ -518,6 +541,7 @@ int uepm_server_destroy(uepm_server* S) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.hk) (void)hipHostFree(sl.hk);
         if (sl.hev) (void)hipHostFree(sl.hev);
+        if (sl.hstat) (void)hipHostFree(sl.hstat);
     }
     for (auto& q : S->plans) uepm_plan_destroy(q.plan);
     delete S;

@@ -32,29 +32,29 @@ hipError_t launch_tridiag_blocked_c64(const TriPlan& tp, const double2* dH, int 
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s);
-hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, double* de, double2* dtau,
-                          hipStream_t s);
+// Failure reporting.  A solve's device status is two words: the number of
+// k-points whose eigen-solve failed and the lowest index among them (all ones
+// when none); launch_status_reset clears it.  Inside a chunk, kstat (one int
+// per k-point of the chunk) carries the tridiagonal stage's flag to the last
+// stage, which adds the chunk's failures (index offset k0) to the status.
+hipError_t launch_status_reset(unsigned long long* status, hipStream_t s);
 // abstol_rel > 0 ends bisection at brackets narrower than abstol_rel ||T||_inf
 // (the precision form, whose T carries complex64 rounding); 0 bisects to the
 // rounding level of each eigenvalue.
 // inv_sweeps: inverse-iteration sweeps for the eigenvectors of T (0: eigenvalues only).
+// A non-finite T gives NaN eigenvalues and vectors and is flagged in kstat
+// (when not null) or else added to status (when not null).
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
-                          double* dW, int inv_sweeps, double abstol_rel, hipStream_t s);
-hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2* dtau, const double* dd,
-                                const double* de, const double* devals, const double* dZ, double2* dX, double* dgrads,
-                                const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
-                                hipStream_t s);
-
-hipError_t launch_backtransform_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dd,
-                                    const double* de, const double* devals, const double* dZ, double2* dX,
-                                    const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
-                                    hipStream_t s);
+                          double* dW, int inv_sweeps, double abstol_rel, int* kstat, unsigned long long* status,
+                          int64_t k0, hipStream_t s);
 // FP64 Rayleigh-Ritz refinement of nbr approximate eigenvectors X (nk x nbr x n,
 // overwritten) against the FP64 Hamiltonians H (nk x n x n column-major):
-// writes the lowest nb refined eigenvalues / vectors / gradients.
+// writes the lowest nb refined eigenvalues / vectors / gradients.  Adds to
+// status the k-points kstat flags, whose Jacobi sweeps do not converge or
+// whose Ritz values are not finite.
 hipError_t launch_refine(const double2* dH, int n, int ld, double2* dX, int nbr, int nb, double* devals,
                          double2* devecs, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                         int64_t nk, hipStream_t s, double2* dY);
+                         int64_t nk, const int* kstat, unsigned long long* status, int64_t k0, hipStream_t s);
 
 // Name of the refinement kernel launch_refine runs for nbr Ritz vectors.
 const char* refine_kernel_name(int nbr);
@@ -70,7 +70,7 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
                                   float2* dtau, float2* work, hipStream_t s);
 // Complex64 register grids after the blocked reduction for n > 256: the
 // trailing size they take over (a panel boundary of the blocked kernel; n
-// itself for n <= 256; 0 with UEPM_REG_SPLIT=0) and the launch on a complex64
+// itself for n <= 256) and the launch on a complex64
 // block left in place by the blocked kernel (arguments as for the FP64 form
 // below; the hand-over workspace is tridiag_reg_work_bytes(m)).
 int tridiag_reg_c64_tail(int n);
@@ -78,8 +78,8 @@ hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int 
                                         int64_t vk, double* dd, double* de, float2* dtau, int64_t dk, float2* work,
                                         hipStream_t s);
 // The same register-resident reduction in FP64 (grids 192 .. 64): trailing
-// size it takes over for a matrix of size n (n itself when n <= 192; 0 with
-// UEPM_F64_REG=0), its hand-over workspace per k-point (bytes), and the
+// size it takes over for a matrix of size n (n itself when n <= 192), its
+// hand-over workspace per k-point (bytes), and the
 // launch on a block of size m whose element (0, 0) sits at H (leading
 // dimension ld, per-k stride hk); reflectors, d, e, tau at the block's offset
 // of the full matrix's outputs (V may alias H: each stage loads its block
@@ -96,9 +96,12 @@ hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, 
 hipError_t launch_cluster_orth(const double* dd, const double* de, const double* devals, double* dZ, int n, int nb,
                                int64_t nk, hipStream_t s);
 // bands_per_group: bands per workgroup (16 lanes each), 4, 8 or 16
+// The FP64 form's back-transform is its last stage: it adds the k-points kstat
+// flags to status (both may be null).
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
-                              int64_t nk, int bands_per_group, hipStream_t s);
+                              int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
+                              int bands_per_group, hipStream_t s);
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
                               int bands_per_group, hipStream_t s);

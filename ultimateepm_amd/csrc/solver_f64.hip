@@ -1,18 +1,16 @@
-// solver_f64.hip — FP64 (complex128) band solve stages on gfx950.
+// solver_f64.hip — assembly and tridiagonal eigen-stage of the band solve on gfx950.
 //
-//   (a) assemble_kernel     H(k) for a batch of k-points (Hamiltonian::SetMatrix +
-//                           the 0.5 (H + H^dagger) step of Diagonalize,
-//                           src/EPP/Hamiltonian.cpp:50-127)
-//   (b) tridiag_kernel      Householder reduction H = Q T Q^H, T real symmetric
-//                           tridiagonal, one workgroup per k
+//   (a) assemble_tiled_kernel / assemble_kernel  H(k) for a batch of k-points
+//                           (Hamiltonian::SetMatrix + the 0.5 (H + H^dagger)
+//                           step of Diagonalize, src/EPP/Hamiltonian.cpp:50-127)
 //   (c) tri_eig_wave_kernel bisection (Sturm counts, multisection) for the lowest
 //                           nb eigenvalues of T and inverse iteration for their
 //                           vectors, one wave per k
-//   (d) backtransform_kernel  cluster re-orthogonalisation, x = Q z, unit norm,
-//                           Hellmann-Feynman gradients (Hamiltonian.cpp:146-194)
 //
-// (b)-(d) replace Eigen::SelfAdjointEigenSolver::compute (Hamiltonian.cpp:129):
-// same eigenvalues to within the backward-stability envelope of a Householder
+// The Householder reduction (b) lives in tridiag_reg.hip / tridiag_blocked.hip
+// and the back-transform (d) in backtransform.hip; (b)-(d) replace
+// Eigen::SelfAdjointEigenSolver::compute (Hamiltonian.cpp:129): same
+// eigenvalues to within the backward-stability envelope of a Householder
 // reduction, eigenvectors up to phase (and basis inside degenerate clusters).
 #include <hip/hip_runtime.h>
 
@@ -24,36 +22,6 @@
 #include "solver.hpp"
 
 namespace uepm {
-
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
-    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// Sum over the workgroup; every thread gets the result. `scratch` holds
-// BS/64 doubles; a __syncthreads() precedes reuse of `scratch`.
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* scratch) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) scratch[w] = v;
-    __syncthreads();
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < BS / 64; ++i) s += scratch[i];
-    return s;
-}
 
 // ------------------------------------------------------------ (a) assembly
 // h_ij and h_ji of the pair (i, j) before hermitisation and spin-orbit: V(G),
@@ -243,107 +211,6 @@ __global__ __launch_bounds__(BS) void assemble_tiled_kernel(DevParams P, const i
     }
 }
 
-// ---------------------------------------------------- (b) tridiagonalisation
-// Unblocked Householder on the full Hermitian matrix held column-major in
-// device memory.  Step j: reflector H_j = I - tau v v^H annihilating
-// A[j+2:, j]; p = tau A22 v; w = p - (tau/2)(p^H v) v; A22 -= v w^H + w v^H.
-// v (v[0] = 1 implied) overwrites A[j+2:, j]; outputs d, e (real), tau.
-template <int BS>
-__global__ __launch_bounds__(BS) void tridiag_kernel(double2* __restrict__ H, int n, int ld, double* __restrict__ dout,
-                                                     double* __restrict__ eout, double2* __restrict__ tout) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double2* v = reinterpret_cast<double2*>(smem);
-    double2* w = v + n;
-    double* red = reinterpret_cast<double*>(w + n);
-    __shared__ double2 s_tau;
-    const int kk = blockIdx.x;
-    double2* A = H + (size_t)kk * ld * n;
-    double* d = dout + (size_t)kk * n;
-    double* e = eout + (size_t)kk * n;
-    double2* tau = tout + (size_t)kk * n;
-    const int tid = threadIdx.x;
-
-    for (int j = 0; j + 1 < n; ++j) {
-        double2* colj = A + (size_t)j * ld;
-        const double2 alpha = colj[j + 1];  // read before the barriers in block_sum
-        double part = 0.0;
-        for (int r = j + 2 + tid; r < n; r += BS) {
-            const double2 x = colj[r];
-            part += x.x * x.x + x.y * x.y;
-        }
-        const double xn2 = block_sum<BS>(part, red);
-        double2 tj = make_double2(0.0, 0.0);
-        double beta = alpha.x;
-        double2 sc = make_double2(1.0, 0.0);
-        if (!(xn2 == 0.0 && alpha.y == 0.0)) {
-            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-            beta = alpha.x >= 0 ? -nrm : nrm;
-            tj = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-            // sc = 1 / (alpha - beta)
-            const double ar = alpha.x - beta, ai = alpha.y;
-            const double den = ar * ar + ai * ai;
-            sc = make_double2(ar / den, -ai / den);
-        }
-        for (int r = j + 2 + tid; r < n; r += BS) {
-            const double2 x = cmul(colj[r], sc);
-            colj[r] = x;
-            v[r] = x;
-        }
-        if (tid == 0) {
-            v[j + 1] = make_double2(1.0, 0.0);
-            colj[j + 1] = make_double2(beta, 0.0);
-            e[j] = beta;
-            tau[j] = tj;
-            d[j] = A[j + (size_t)j * ld].x;
-        }
-        __syncthreads();
-        if (tj.x == 0.0 && tj.y == 0.0) continue;
-        // p = tau * A22 v
-        for (int r = j + 1 + tid; r < n; r += BS) {
-            double2 acc = make_double2(0.0, 0.0);
-            for (int c = j + 1; c < n; ++c) {
-                const double2 a = A[r + (size_t)c * ld];
-                const double2 vc = v[c];
-                acc.x += a.x * vc.x - a.y * vc.y;
-                acc.y += a.x * vc.y + a.y * vc.x;
-            }
-            w[r] = cmul(tj, acc);
-        }
-        __syncthreads();
-        double pr = 0.0, pi = 0.0;
-        for (int r = j + 1 + tid; r < n; r += BS) {
-            const double2 q = cmulc(w[r], v[r]);
-            pr += q.x;
-            pi += q.y;
-        }
-        pr = block_sum<BS>(pr, red);
-        __syncthreads();
-        pi = block_sum<BS>(pi, red);
-        // al = -0.5 * tau * (p^H v)
-        const double2 al = cmul(make_double2(-0.5 * tj.x, -0.5 * tj.y), make_double2(pr, pi));
-        for (int r = j + 1 + tid; r < n; r += BS) w[r] = cadd(w[r], cmul(al, v[r]));
-        __syncthreads();
-        // A22 -= v w^H + w v^H
-        for (int r = j + 1 + tid; r < n; r += BS) {
-            const double2 vr = v[r], wr = w[r];
-            for (int c = j + 1; c < n; ++c) {
-                const double2 vc = v[c], wc = w[c];
-                double2 a = A[r + (size_t)c * ld];
-                // v_r conj(w_c) + w_r conj(v_c)
-                a.x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
-                a.y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
-                A[r + (size_t)c * ld] = a;
-            }
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        d[n - 1] = A[(n - 1) + (size_t)(n - 1) * ld].x;
-        e[n - 1] = 0.0;
-        tau[n - 1] = make_double2(0.0, 0.0);
-    }
-}
-
 // ------------------------------------- (c) tridiagonal eigenvalues / vectors
 // Start vector of inverse iteration: a fixed pseudo-random value per (row, band).
 __device__ __forceinline__ double lane_rand(uint32_t i, uint32_t b) {
@@ -409,7 +276,8 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
                                                           const double* __restrict__ eall, int n, int nb, int64_t nk,
                                                           double* __restrict__ evals, double* __restrict__ Z,
                                                           double* __restrict__ W, int inv_sweeps,
-                                                          double abstol_rel) {
+                                                          double abstol_rel, int* __restrict__ kstat,
+                                                          unsigned long long* __restrict__ status, int64_t k0) {
     constexpr int NBW = 64 / S;  // bands per wave
     const int64_t kk = blockIdx.x;
     const int lane = threadIdx.x, sub = lane % S;
@@ -419,13 +287,35 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
     const double* e = eall + kk * n;
     // Gershgorin interval and scale (wave-uniform)
     double lo = DBL_MAX, hi = -DBL_MAX, emax2 = 0.0, tnorm = 0.0;
+    bool finite = true;
     for (int i = 0; i < n; ++i) {
         const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
         const double er = i + 1 < n ? fabs(e[i]) : 0.0;
+        finite = finite && isfinite(d[i]) && isfinite(er);
         lo = fmin(lo, d[i] - el - er);
         hi = fmax(hi, d[i] + el + er);
         emax2 = fmax(emax2, er * er);
         tnorm = fmax(tnorm, fabs(d[i]) + el + er);
+    }
+    // A non-finite T (a non-finite k-point, or H(k) overflowing) has no
+    // eigen-decomposition (the reference's Eigen solver reports NoConvergence,
+    // Hamiltonian.cpp:130-133): NaN eigenvalues and vectors, and the k-point is
+    // flagged -- in kstat for a later stage to report, or in status directly
+    // when this is the last stage.  Both conditions are the same in every wave.
+    if (blockIdx.y == 0 && lane == 0) {
+        if (kstat) kstat[kk] = finite ? 0 : 1;
+        else if (status && !finite) {
+            atomicAdd(&status[0], 1ull);
+            atomicMin(&status[1], (unsigned long long)(k0 + kk));
+        }
+    }
+    if (!finite) {
+        if (live && sub == 0) {
+            evals[kk * nb + b] = __builtin_nan("");
+            if (inv_sweeps > 0)
+                for (int i = 0; i < n; ++i) Z[(size_t)(kk * nb + b) * n + i] = __builtin_nan("");
+        }
+        return;
     }
     const double pivmin = DBL_MIN * fmax(1.0, emax2);
     const double eps = DBL_EPSILON;
@@ -612,129 +502,8 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
 #undef IDX
 }
 
-// ------------------------------------------- (d) back-transform + gradients
-// One workgroup per k, 4 waves; wave w owns bands b = w, w+4, ...
-// Z: nk x nb x n real eigenvectors of T (unit norm).  X (output, complex):
-// nk x nb x dim.  Bands within ortol of each other are re-orthogonalised
-// (modified Gram-Schmidt, ascending) before x = Q z.
-__device__ __forceinline__ double2 to_d2(double2 a) { return a; }
-__device__ __forceinline__ double2 to_d2(float2 a) { return make_double2((double)a.x, (double)a.y); }
-
-// RT: reflector storage type (double2 for the FP64 path, float2 for the
-// complex64 reduction of the mixed-precision path); arithmetic is FP64.
-template <int BS, typename RT>
-__global__ __launch_bounds__(BS) void backtransform_kernel(const RT* __restrict__ H, int n, int ld,
-                                                           const RT* __restrict__ tauall,
-                                                           const double* __restrict__ dall,
-                                                           const double* __restrict__ eall,
-                                                           const double* __restrict__ evals, const double* Z,
-                                                           double2* __restrict__ X, double* __restrict__ grads,
-                                                           DevParams P, const double* __restrict__ kpts,
-                                                           const int32_t* __restrict__ G, int nb) {
-    const int kk = blockIdx.x;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    constexpr int NW = BS / 64;
-    const RT* A = H + (size_t)kk * ld * n;
-    const RT* tau = tauall + (size_t)kk * n;
-    const double* z = Z + (size_t)kk * nb * n;
-    double2* x = X + (size_t)kk * nb * n;
-    const double* ev = evals + (size_t)kk * nb;
-    // ||T||_1 for the cluster tolerance
-    double tn = 0.0;
-    {
-        const double* d = dall + (size_t)kk * n;
-        const double* e = eall + (size_t)kk * n;
-        for (int i = lane; i < n; i += 64) {
-            const double el = i > 0 ? fabs(e[i - 1]) : 0.0;
-            const double er = i + 1 < n ? fabs(e[i]) : 0.0;
-            tn = fmax(tn, fabs(d[i]) + el + er);
-        }
-        for (int o = 32; o > 0; o >>= 1) tn = fmax(tn, __shfl_xor(tn, o, 64));
-    }
-    const double ortol = 1e-3 * tn;
-    // Copy z -> x (real), MGS within clusters.  Serial over bands in wave 0.
-    if (wv == 0) {
-        int cstart = 0;
-        for (int b = 0; b < nb; ++b) {
-            if (b > 0 && ev[b] - ev[b - 1] > ortol) cstart = b;
-            double2* xb = x + (size_t)b * n;
-            for (int i = lane; i < n; i += 64) xb[i] = make_double2(z[(size_t)b * n + i], 0.0);
-            for (int c = cstart; c < b; ++c) {
-                const double2* xc = x + (size_t)c * n;
-                double s = 0.0;
-                for (int i = lane; i < n; i += 64) s += xc[i].x * xb[i].x;
-                s = wave_sum(s);
-                for (int i = lane; i < n; i += 64) xb[i].x -= s * xc[i].x;
-            }
-            double nn = 0.0;
-            for (int i = lane; i < n; i += 64) nn += xb[i].x * xb[i].x;
-            nn = wave_sum(nn);
-            const double inv = 1.0 / sqrt(nn);
-            for (int i = lane; i < n; i += 64) xb[i].x *= inv;
-        }
-    }
-    __syncthreads();
-    // x = H_0 H_1 ... H_{n-2} z: apply H_{n-2} first.
-    for (int b = wv; b < nb; b += NW) {
-        double2* xb = x + (size_t)b * n;
-        for (int j = n - 2; j >= 0; --j) {
-            const double2 tj = to_d2(tau[j]);
-            if (tj.x == 0.0 && tj.y == 0.0) continue;
-            const RT* vj = A + (size_t)j * ld;  // v at rows j+1..n-1, v[j+1] = 1
-            const int r0 = lane + (j + 1 > lane ? ((j + 1 - lane + 63) >> 6) << 6 : 0);
-            double sr = 0.0, si = 0.0;
-            for (int r = r0; r < n; r += 64) {
-                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : to_d2(vj[r]);
-                const double2 q = cmulc(vr, xb[r]);
-                sr += q.x;
-                si += q.y;
-            }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            const double2 s = cmul(tj, make_double2(sr, si));
-            for (int r = r0; r < n; r += 64) {
-                const double2 vr = (r == j + 1) ? make_double2(1.0, 0.0) : to_d2(vj[r]);
-                xb[r] = csub(xb[r], cmul(s, vr));
-            }
-        }
-        __threadfence_block();  // rows i + N were written by other lanes
-        if (grads) {
-            // Hellmann-Feynman on the kinetic term (Hamiltonian.cpp:146-194)
-            const double* k = kpts + 3 * (size_t)kk;
-            const int N = P.N;
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            for (int i = lane; i < N; i += 64) {
-                double wt = xb[i].x * xb[i].x + xb[i].y * xb[i].y;
-                if (P.soc) wt += xb[i + N].x * xb[i + N].x + xb[i + N].y * xb[i + N].y;
-                sx += (k[0] + (double)G[3 * i]) * wt;
-                sy += (k[1] + (double)G[3 * i + 1]) * wt;
-                sz += (k[2] + (double)G[3 * i + 2]) * wt;
-            }
-            sx = wave_sum(sx);
-            sy = wave_sum(sy);
-            sz = wave_sum(sz);
-            if (lane == 0) {
-                double* g = grads + 3 * ((size_t)kk * nb + b);
-                g[0] = sx * P.grad_pref * P.frac_to_phys;
-                g[1] = sy * P.grad_pref * P.frac_to_phys;
-                g[2] = sz * P.grad_pref * P.frac_to_phys;
-            }
-        }
-    }
-}
-
 // ------------------------------------------------------------ launchers
 constexpr int kAsmBS = 256;
-// UEPM_ASM_V1=1 keeps the one-element-per-thread assembly for every plan (A/B)
-static bool assemble_v1() {
-    static const bool on = [] {
-        const char* v = std::getenv("UEPM_ASM_V1");
-        return v && v[0] == '1';
-    }();
-    return on;
-}
-constexpr int kTriBS = 256;
-constexpr int kBtBS = 256;
 
 hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* dk, int64_t nk, double2* dH, int ld,
                            hipStream_t s) {
@@ -742,7 +511,7 @@ hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* 
     // kernel's transpose slots): up to 766 vectors (860 with spin-orbit) in
     // the 160 KB of a CU
     const size_t shm = sizeof(KState) * (size_t)P.N;
-    const bool tiled = !P.soc && !assemble_v1();
+    const bool tiled = !P.soc;
     const size_t fixed = sizeof(double2) * kVPhaseTab + (tiled ? sizeof(double2) * (kAsmBS / 64) * 16 * 17 : 0);
     if (shm + fixed > 163840) return hipErrorInvalidValue;
     auto kern = tiled ? assemble_tiled_kernel<kAsmBS> : assemble_kernel<kAsmBS>;
@@ -752,42 +521,30 @@ hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* 
     return hipGetLastError();
 }
 
-hipError_t launch_tridiag(double2* dH, int n, int ld, int64_t nk, double* dd, double* de, double2* dtau,
-                          hipStream_t s) {
-    const size_t shm = sizeof(double2) * 2 * (size_t)n + sizeof(double) * (kTriBS / 64);
-    hipLaunchKernelGGL(tridiag_kernel<kTriBS>, dim3((unsigned)nk), dim3(kTriBS), shm, s, dH, n, ld, dd, de, dtau);
-    return hipGetLastError();
-}
-
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
-                          double* dW, int inv_sweeps, double abstol_rel, hipStream_t s) {
+                          double* dW, int inv_sweeps, double abstol_rel, int* kstat, unsigned long long* status,
+                          int64_t k0, hipStream_t s) {
     if (nb <= 16)
         hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, inv_sweeps, abstol_rel);
+                           dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     else if (nb <= 32)
         hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
-                           dZ, dW, inv_sweeps, abstol_rel);
+                           dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     else
         hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), 0, s, dd,
-                           de, n, nb, nk, devals, dZ, dW, inv_sweeps, abstol_rel);
+                           de, n, nb, nk, devals, dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     return hipGetLastError();
 }
 
-hipError_t launch_backtransform(const double2* dH, int n, int ld, const double2* dtau, const double* dd,
-                                const double* de, const double* devals, const double* dZ, double2* dX, double* dgrads,
-                                const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
-                                hipStream_t s) {
-    hipLaunchKernelGGL((backtransform_kernel<kBtBS, double2>), dim3((unsigned)nk), dim3(kBtBS), 0, s, dH, n, ld, dtau,
-                       dd, de, devals, dZ, dX, dgrads, P, dk, dG, nb);
-    return hipGetLastError();
+// status[0] = 0 failed k-points, status[1] = no first index (a kernel node, so
+// that a solve captured into a graph clears it on every replay)
+__global__ void status_reset_kernel(unsigned long long* status) {
+    status[0] = 0ull;
+    status[1] = ~0ull;
 }
 
-hipError_t launch_backtransform_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dd,
-                                    const double* de, const double* devals, const double* dZ, double2* dX,
-                                    const DevParams& P, const double* dk, const int32_t* dG, int nb, int64_t nk,
-                                    hipStream_t s) {
-    hipLaunchKernelGGL((backtransform_kernel<kBtBS, float2>), dim3((unsigned)nk), dim3(kBtBS), 0, s, dV, n, ld, dtau,
-                       dd, de, devals, dZ, dX, nullptr, P, dk, dG, nb);
+hipError_t launch_status_reset(unsigned long long* status, hipStream_t s) {
+    hipLaunchKernelGGL(status_reset_kernel, dim3(1), dim3(1), 0, s, status);
     return hipGetLastError();
 }
 

@@ -477,15 +477,7 @@ int tridiag_reg_max_n() { return 256; }
 // 64 (2 x 4).  Register budgets (workgroups per CU): 256 and 224
 // low-register, 256 and 223 VGPRs (1); 192 and 160 low-register, 128 VGPRs
 // (2); 128 at 128 VGPRs (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
-// UEPM_REG_SPLIT=0 keeps one grid.
 namespace {
-int reg_split_mode() {
-    static const int mode = [] {
-        const char* v = std::getenv("UEPM_REG_SPLIT");
-        return v ? std::atoi(v) : 1;
-    }();
-    return mode;
-}
 constexpr int kNumGrids = 7;
 constexpr int kGrid[kNumGrids] = {256, 224, 192, 160, 128, 96, 64};
 // index of the smallest grid that holds n
@@ -497,7 +489,6 @@ int grid_of(int n) {
 }  // namespace
 
 int tridiag_reg_split(int n) {
-    if (reg_split_mode() == 0) return 0;
     const int g = grid_of(n);
     return g < kNumGrids - 1 ? kGrid[g + 1] : 0;
 }
@@ -574,7 +565,7 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
 }
 
 int tridiag_reg_c64_tail(int n) {
-    if (reg_split_mode() == 0 || n < 2) return 0;
+    if (n < 2) return 0;
     if (n <= kGrid[0]) return n;
     const int P = tridiag_blocked_panel();
     const int s0 = ((n - kGrid[0] + P - 1) / P) * P;
@@ -604,13 +595,6 @@ int split64(int n) {
     const int g = grid64_of(n);
     return g < kNumGrids64 - 1 ? kGrid64[g + 1] : 0;
 }
-bool f64_reg_enabled() {
-    static const bool on = [] {
-        const char* v = std::getenv("UEPM_F64_REG");
-        return !(v && v[0] == '0');
-    }();
-    return on;
-}
 void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv, int64_t vk,
                   double* dd, double* de, double2* dtau, int64_t dk, int s_end, double2* B, int ldb, hipStream_t s) {
     switch (g) {
@@ -624,7 +608,7 @@ void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk
 }  // namespace
 
 int tridiag_reg_c128_tail(int n) {
-    if (!f64_reg_enabled() || n < 2) return 0;
+    if (n < 2) return 0;
     if (n <= kGrid64[0]) return n;
     // the blocked reduction stops at a panel boundary s0 >= n - 128
     const int P = tridiag_blocked_panel();

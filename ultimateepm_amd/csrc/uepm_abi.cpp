@@ -61,7 +61,7 @@ int mixed_c64_tail(const uepm_plan* p) {
 }
 
 struct Sizes {
-    size_t H, d, e, tau, Vf, tauf, S32, Ev, X0, Y, S, Z, W, X;
+    size_t H, d, e, tau, Vf, tauf, S32, Ev, X0, S, Z, W, X, kstat;
 };
 Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
     const int n = p->dim, nb = p->nb;
@@ -82,8 +82,7 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
                                  chunk;
         z.Ev = sizeof(double) * (size_t)nbr * chunk;
         z.X0 = sizeof(double2) * (size_t)n * nbr * chunk;
-        if (nbr > 16) z.Y = sizeof(double2) * (size_t)n * nbr * chunk;
-    } else if (!p->tri_v1) {
+    } else {
         // blocked-reduction scratch, then (after it) the FP64 register grids' hand-over blocks
         const int t = uepm::tridiag_reg_c128_tail(n);
         const size_t blocked = t < n ? sizeof(double2) * (size_t)p->tp.scratch_elems : 0;
@@ -94,6 +93,7 @@ Sizes plan_sizes(const uepm_plan* p, int64_t chunk) {
         z.W = sizeof(double) * 4 * (size_t)n * nbr * chunk;  // U factor (3 rows) + iterate
         if (!mixed) z.X = sizeof(double2) * (size_t)n * nb * chunk;
     }
+    z.kstat = sizeof(int) * (size_t)chunk;
     return z;
 }
 }  // namespace
@@ -103,13 +103,24 @@ int plan_lanes(const uepm_plan* p, int64_t nk) { return (p->cfg.lanes == 1 || nk
 size_t plan_workspace_bytes(const uepm_plan* p, int64_t chunk, int lanes) {
     const Sizes z = plan_sizes(p, chunk);
     return (size_t)lanes *
-           (z.H + z.d + z.e + z.tau + z.Vf + z.tauf + z.S32 + z.Ev + z.X0 + z.Y + z.S + z.Z + z.W + z.X);
+           (z.H + z.d + z.e + z.tau + z.Vf + z.tauf + z.S32 + z.Ev + z.X0 + z.S + z.Z + z.W + z.X + z.kstat);
 }
 
 int plan_reserve(uepm_plan* p, int64_t chunk, int lanes) {
     const Sizes z = plan_sizes(p, chunk);
     for (int l = 0; l < lanes; ++l) {
         Lane& w = p->lane[l];
+        if (l == 0 && p->pins > 0) {
+            // a batch graph holds lane 0's buffers: they may be used, not reallocated
+            const DeviceBuffer* b[] = {&w.H, &w.d, &w.e, &w.tau, &w.Vf, &w.tauf, &w.S32, &w.Ev, &w.X0,
+                                       &w.S, &w.Z, &w.W, &w.X, &w.kstat};
+            const size_t need[] = {z.H, z.d, z.e, z.tau, z.Vf, z.tauf, z.S32, z.Ev, z.X0, z.S, z.Z, z.W, z.X, z.kstat};
+            for (int i = 0; i < 14; ++i)
+                if (need[i] > b[i]->bytes)
+                    return fail(UEPM_ERR_INVALID_ARG,
+                                "the plan's workspace is held by a batch graph: solves of up to the reserved size "
+                                "only, until uepm_batch_destroy");
+        }
         HIP_OK(w.H.ensure(z.H));
         HIP_OK(w.d.ensure(z.d));
         HIP_OK(w.e.ensure(z.e));
@@ -119,7 +130,7 @@ int plan_reserve(uepm_plan* p, int64_t chunk, int lanes) {
         if (z.S32) HIP_OK(w.S32.ensure(z.S32));
         if (z.Ev) HIP_OK(w.Ev.ensure(z.Ev));
         if (z.X0) HIP_OK(w.X0.ensure(z.X0));
-        if (z.Y) HIP_OK(w.Y.ensure(z.Y));
+        HIP_OK(w.kstat.ensure(z.kstat));
         if (z.S) HIP_OK(w.S.ensure(z.S));
         if (z.Z) HIP_OK(w.Z.ensure(z.Z));
         if (z.W) HIP_OK(w.W.ensure(z.W));
@@ -130,16 +141,9 @@ int plan_reserve(uepm_plan* p, int64_t chunk, int lanes) {
 
 void plan_apply_config(uepm_plan* p) {
     const bool mixed = p->flags & UEPM_MIXED;
-    const char* reg = std::getenv("UEPM_TRIDIAG_REG");
     const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
-    if (p->cfg.tridiag == 2) p->tri_reg = reg_ok;
-    else if (p->cfg.tridiag == 1) p->tri_reg = false;
-    else p->tri_reg = reg_ok && !(reg && reg[0] == '0');
-    int bs = p->cfg.tri_block;
-    if (bs == 0) {
-        const char* bs_env = std::getenv("UEPM_TRI_BS");
-        bs = (bs_env && std::atoi(bs_env) == 256) ? 256 : 512;
-    }
+    p->tri_reg = p->cfg.tridiag != 1 && reg_ok;
+    const int bs = p->cfg.tri_block == 256 ? 256 : 512;
     p->tp = uepm::tri_plan(p->dim, (int)sizeof(double2), p->ctx->lds_limit, bs);
     p->tp32 = uepm::tri_plan(p->dim, (int)sizeof(float2), p->ctx->lds_limit, bs);
     p->bt_bands = p->cfg.bt_bands > 0 ? p->cfg.bt_bands : 16;
@@ -394,13 +398,14 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     P.grad_pref = 2.0 * diag_factor * (P.tpa * P.tpa);
     P.frac_to_phys = 1.0 / P.tpa;
 
-    const char* v1 = std::getenv("UEPM_TRIDIAG_V1");
-    p->tri_v1 = v1 && v1[0] == '1';
-    const char* btv1 = std::getenv("UEPM_BT_V1");
-    p->bt_v1 = btv1 && btv1[0] == '1';
     plan_apply_config(p);
     hipError_t e = p->G.ensure(sizeof(int32_t) * 3 * (size_t)N);
     if (e == hipSuccess) e = hipMemcpy(p->G.p, G, sizeof(int32_t) * 3 * (size_t)N, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = p->status.ensure(2 * sizeof(unsigned long long));
+    if (e == hipSuccess) {
+        const unsigned long long none[2] = {0ull, ~0ull};
+        e = hipMemcpy(p->status.p, none, sizeof(none), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         delete p;
         return fail(UEPM_ERR_DEVICE, std::string("uepm_plan_create: ") + hipGetErrorString(e));
@@ -410,6 +415,8 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
 }
 
 int uepm_plan_destroy(uepm_plan* plan) {
+    if (plan && plan->pins > 0)
+        return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_destroy: the plan is used by a batch graph (uepm_batch_destroy first)");
     delete plan;
     return UEPM_OK;
 }
@@ -439,6 +446,7 @@ int uepm_plan_get_config(const uepm_plan* p, uepm_config* cfg) {
 
 int uepm_plan_set_config(uepm_plan* p, const uepm_config* cfg) {
     if (!p || !cfg) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_set_config: null argument");
+    if (p->pins > 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_set_config: the plan is used by a batch graph");
     if (cfg->tridiag < 0 || cfg->tridiag > 2)
         return fail(UEPM_ERR_INVALID_ARG, "uepm_config.tridiag must be 0, 1 or 2");
     if (cfg->tridiag == 2 && !((p->flags & UEPM_MIXED) && p->dim <= uepm::tridiag_reg_max_n()))
@@ -472,6 +480,7 @@ int uepm_plan_reserve(uepm_plan* p, int64_t nk) {
 
 int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, uepm_config* chosen) {
     if (!p || !dk || nk <= 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_tune: null plan, no k-points");
+    if (p->pins > 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_tune: the plan is used by a batch graph");
     HIP_OK(hipSetDevice(p->ctx->device));
     hipStream_t s = (hipStream_t)stream_;
     // candidates: reduction variant x workgroup size, back-transform band groups
@@ -565,15 +574,25 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
     const int64_t chunk = plan_chunk(p, nk);
     int lanes = plan_lanes(p, nk);
-    if (lanes > 1) {  // a stream under graph capture keeps one lane
+    bool capturing = false;
+    {
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) lanes = 1;
+        capturing = hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
     }
+    if (capturing) lanes = 1;  // a stream under graph capture keeps one lane
     {
         const int st = plan_reserve(p, chunk, lanes);
         if (st != UEPM_OK) return st;
     }
     const int nchunks = (int)((nk + chunk - 1) / chunk);
+    // after the plan's previous solve, whichever stream it ran on (the
+    // workspace and the status words are the plan's)
+    if (!capturing) {
+        if (!p->lane_done[0]) HIP_OK(hipEventCreateWithFlags(&p->lane_done[0], hipEventDisableTiming));
+        else HIP_OK(hipStreamWaitEvent(s, p->lane_done[0], 0));
+    }
+    unsigned long long* status = p->status.as<unsigned long long>();
+    HIP_OK(uepm::launch_status_reset(status, s));
     if (lanes > 1) {
         // the side stream starts after the caller's earlier work and the caller's
         // stream resumes after the side stream's last chunk (join below)
@@ -600,13 +619,11 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
     p->stage_name[kTridiag] = mixed ? (p->tri_reg              ? "tridiag_reg_kernel<c64>"
                                        : mixed_c64_tail(p) > 0 ? "tridiag_blocked+reg_kernel<c64>"
                                                                : "tridiag_blocked_kernel<c64>")
-                                    : p->tri_v1 ? "tridiag_kernel<c128>"
                                     : uepm::tridiag_reg_c128_tail(n) == n ? "tridiag_reg_kernel<c128>"
                                     : uepm::tridiag_reg_c128_tail(n) > 0 ? "tridiag_blocked+reg_kernel<c128>"
                                                                          : "tridiag_blocked_kernel<c128>";
     p->stage_name[kTriEig] = "tri_eig_wave_kernel";
-    p->stage_name[kBack] = p->bt_v1 ? "backtransform_kernel"
-                                    : (mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>");
+    p->stage_name[kBack] = mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>";
     p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
     // the chunks; on an error the side stream is still joined into the caller's
     auto run_chunks = [&]() -> int {
@@ -617,6 +634,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             const int64_t c = std::min(chunk, nk - c0);
             const double* kc = dk + 3 * c0;
             double2* Hc = w.H.as<double2>();
+            int* ks = w.kstat.as<int>();
             HIP_OK(rec(ci, kAssemble, 0));
             HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
             HIP_OK(rec(ci, kAssemble, 1));
@@ -649,36 +667,27 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
                 // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
                 // for inverse iteration; the FP64 refinement sets the eigenvalues
                 HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nbr, c, w.Ev.as<double>(),
-                                            w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol,
-                                            cs));
+                                            w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol, ks,
+                                            nullptr, c0, cs));
                 HIP_OK(rec(ci, kTriEig, 1));
                 HIP_OK(rec(ci, kBack, 0));
-                if (p->bt_v1) {
-                    HIP_OK(uepm::launch_backtransform_c64(Vc, n, n, w.tauf.as<float2>(), w.d.as<double>(),
-                                                          w.e.as<double>(), w.Ev.as<double>(), w.Z.as<double>(),
-                                                          w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), nbr, c, cs));
-                } else {
-                    HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
-                                                     w.Z.as<double>(), n, nbr, c, cs));
-                    HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
-                                                    w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands,
-                                                    cs));
-                }
+                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
+                                                 w.Z.as<double>(), n, nbr, c, cs));
+                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
+                                                w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands, cs));
                 HIP_OK(rec(ci, kBack, 1));
                 HIP_OK(rec(ci, kRefine, 0));
                 HIP_OK(uepm::launch_refine(Hc, n, n, w.X0.as<double2>(), nbr, nb, devals + c0 * nb,
                                            devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
-                                           dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, cs,
-                                           w.Y.as<double2>()));
+                                           dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, ks,
+                                           status, c0, cs));
                 HIP_OK(rec(ci, kRefine, 1));
                 continue;
             }
             HIP_OK(rec(ci, kTridiag, 0));
-            if (p->tri_v1)
-                HIP_OK(uepm::launch_tridiag(Hc, n, n, c, w.d.as<double>(), w.e.as<double>(), w.tau.as<double2>(), cs));
-            else {
+            {
                 // the blocked reduction for the leading steps, the FP64 register grids for the
-                // trailing block of size t (the whole matrix when n <= 128); reflectors in place in H
+                // trailing block of size t (the whole matrix when n <= 192); reflectors in place in H
                 const int t = uepm::tridiag_reg_c128_tail(n);
                 const int s0 = n - t;
                 const int64_t nn = (int64_t)n * n;
@@ -697,23 +706,17 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
             HIP_OK(rec(ci, kTriEig, 0));
             HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nb, c, devals + c0 * nb,
                                         need_vec ? w.Z.as<double>() : nullptr, need_vec ? w.W.as<double>() : nullptr,
-                                        need_vec ? kFp64InvSweeps : 0, 0.0, cs));
+                                        need_vec ? kFp64InvSweeps : 0, 0.0, need_vec ? ks : nullptr,
+                                        need_vec ? nullptr : status, c0, cs));
             HIP_OK(rec(ci, kTriEig, 1));
             if (need_vec) {
                 double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X.as<double2>();
                 HIP_OK(rec(ci, kBack, 0));
-                if (p->bt_v1) {
-                    HIP_OK(uepm::launch_backtransform(Hc, n, n, w.tau.as<double2>(), w.d.as<double>(),
-                                                      w.e.as<double>(), devals + c0 * nb, w.Z.as<double>(), Xc,
-                                                      dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                      p->G.as<int32_t>(), nb, c, cs));
-                } else {
-                    HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
-                                                     w.Z.as<double>(), n, nb, c, cs));
-                    HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
-                                                    dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc,
-                                                    p->G.as<int32_t>(), c, p->bt_bands, cs));
-                }
+                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
+                                                 w.Z.as<double>(), n, nb, c, cs));
+                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
+                                                dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(),
+                                                c, ks, status, c0, p->bt_bands, cs));
                 HIP_OK(rec(ci, kBack, 1));
             }
         }
@@ -724,6 +727,7 @@ int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals
         HIP_OK(hipEventRecord(p->join, p->side));
         HIP_OK(hipStreamWaitEvent(s, p->join, 0));
     }
+    if (!capturing) HIP_OK(hipEventRecord(p->lane_done[0], s));
     if (rc != UEPM_OK) return rc;
     if (p->timing) {
         HIP_OK(hipStreamSynchronize(s));
@@ -831,6 +835,9 @@ int uepm_solve(uepm_plan* p, const double* k, int64_t nk, double* evals, double*
     if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: nk < 0");
     if (nk == 0) return UEPM_OK;
     if (!k || !evals) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: null k or eigenvalue buffer");
+    for (int64_t i = 0; i < 3 * nk; ++i)
+        if (!std::isfinite(k[i]))
+            return fail(UEPM_ERR_INVALID_ARG, "uepm_solve: k-point " + std::to_string(i / 3) + " is not finite");
     HIP_OK(hipSetDevice(p->ctx->device));
     const int n = p->dim, nb = p->nb;
     DeviceBuffer dk, dev, dvec, dgr;
@@ -846,6 +853,20 @@ int uepm_solve(uepm_plan* p, const double* k, int64_t nk, double* evals, double*
     HIP_OK(hipMemcpy(evals, dev.p, sizeof(double) * nb * nk, hipMemcpyDeviceToHost));
     if (evecs) HIP_OK(hipMemcpy(evecs, dvec.p, sizeof(double2) * (size_t)n * nb * nk, hipMemcpyDeviceToHost));
     if (grads) HIP_OK(hipMemcpy(grads, dgr.p, sizeof(double) * 3 * nb * nk, hipMemcpyDeviceToHost));
+    return uepm_plan_status(p, nullptr, nullptr, nullptr);
+}
+
+int uepm_plan_status(uepm_plan* p, void* stream, int64_t* n_failed, int64_t* first_failed) {
+    if (!p) return fail(UEPM_ERR_INVALID_ARG, "uepm_plan_status: null plan");
+    HIP_OK(hipSetDevice(p->ctx->device));
+    unsigned long long h[2] = {0ull, ~0ull};
+    HIP_OK(hipMemcpyAsync(h, p->status.p, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_OK(hipStreamSynchronize((hipStream_t)stream));
+    if (n_failed) *n_failed = (int64_t)h[0];
+    if (first_failed) *first_failed = h[0] ? (int64_t)h[1] : -1;
+    if (h[0])
+        return fail(UEPM_ERR_NOT_CONVERGED, "Eigenvalue decomposition failed for " + std::to_string(h[0]) +
+                                                " k-point(s), the first at index " + std::to_string(h[1]));
     return UEPM_OK;
 }
 

@@ -68,7 +68,8 @@ constexpr int kMixedInvSweeps = 2;
 // stream, each with its own workspace.
 struct Lane {
     DeviceBuffer H, d, e, tau, Z, W, X, S;
-    DeviceBuffer Vf, tauf, S32, Ev, X0, Y;  // precision form
+    DeviceBuffer Vf, tauf, S32, Ev, X0;  // precision form
+    DeviceBuffer kstat;                  // per k-point failure flag of the chunk in flight
 };
 
 }  // namespace uepm_host
@@ -88,11 +89,18 @@ struct uepm_plan {
     uepm_host::Lane lane[2];
     hipStream_t side = nullptr;  // the second lane's stream (created on first use)
     hipEvent_t fork = nullptr, join = nullptr;
+    // the last solve's failure count and first failed k-point (device, two
+    // words, cleared by each solve)
+    uepm_host::DeviceBuffer status;
+    // batch graphs holding raw pointers into lane 0's workspace: while > 0,
+    // lane 0 is not reallocated and the plan is not destroyed
+    int pins = 0;
+    // completion of each lane's last chunk: the next solve's streams wait on
+    // them, so solves of one plan on different streams do not overlap
+    hipEvent_t lane_done[2] = {nullptr, nullptr};
     uepm::TriPlan tp{}, tp32{};
     uepm_config cfg{};    // requested configuration (0 fields = automatic)
-    bool tri_v1 = false;  // the unblocked full-matrix reduction (A/B reference)
-    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 192)
-    bool bt_v1 = false;   // the one-wave-per-band back-transform (A/B reference)
+    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 256)
     int bt_bands = 16;    // bands per back-transform workgroup
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
@@ -102,6 +110,8 @@ struct uepm_plan {
     ~uepm_plan() {
         for (auto e : ev) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
+        for (auto e : lane_done)
+            if (e) (void)hipEventDestroy(e);
         if (join) (void)hipEventDestroy(join);
         if (side) (void)hipStreamDestroy(side);
     }

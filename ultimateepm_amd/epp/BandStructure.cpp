@@ -1,6 +1,7 @@
 #include "BandStructure.h"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <fstream>
 #include <iostream>
@@ -80,6 +81,11 @@ void BandStructure::solve(bool compute_gradient, bool clamp_rows) {
     m_energies_gradient.clear();
     if (compute_gradient) m_energies_gradient.assign(nk, std::vector<Vector3D<double>>(m_nb_bands));
     if (nk == 0 || nb == 0) return;
+    // a non-finite k-point has no eigen-decomposition: the reference's Eigen
+    // solver reports NoConvergence there (Hamiltonian.cpp:130-133)
+    for (const auto& kp : m_kpoints)
+        if (!std::isfinite(kp.X) || !std::isfinite(kp.Y) || !std::isfinite(kp.Z))
+            throw std::runtime_error("Eigenvalue decomposition failed");
 
     const uepm_material mat = m_material.to_uepm();
     std::vector<int32_t> g(3 * (size_t)N);

@@ -1,5 +1,6 @@
 #include "Hamiltonian.h"
 
+#include <cmath>
 #include <stdexcept>
 
 #include "device_context.h"
@@ -52,6 +53,10 @@ std::vector<int32_t> flat_basis(const std::vector<Vector3D<int>>& G) {
 }  // namespace
 
 void Hamiltonian::Diagonalize(bool keep_eigenvectors) {
+    // a non-finite k-point has no eigen-decomposition: the reference's Eigen
+    // solver reports NoConvergence there (Hamiltonian.cpp:130-133)
+    if (!std::isfinite(m_k.X) || !std::isfinite(m_k.Y) || !std::isfinite(m_k.Z))
+        throw std::runtime_error("Eigenvalue decomposition failed");
     const int N = static_cast<int>(m_basisVectors.size());
     const int dim = m_soc ? 2 * N : N;
     const int slot = (m_nl ? 1 : 0) | (m_soc ? 2 : 0) | (keep_eigenvectors ? 4 : 0);
