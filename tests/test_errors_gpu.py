@@ -99,7 +99,7 @@ def test_overflowing_k_reports_not_converged(gpu_ctx, oracle):
     G = api.basis_vectors(6)
     ks = api.bz_sample(8, 6)
     ks[3] = (1e160, 0.0, 0.0)
-    p = api.Plan(gpu_ctx, mat, G, api.NONLOCAL | api.GRADIENTS, 8)
+    p = api.Plan(gpu_ctx, mat, G, api.NONLOCAL | api.VECTORS | api.GRADIENTS, 8)
     with pytest.raises(UepmError) as e:
         p.solve(ks)
     assert e.value.code == UEPM_ERR_NOT_CONVERGED and "first at index 3" in str(e.value)
@@ -109,6 +109,7 @@ def test_overflowing_k_reports_not_converged(gpu_ctx, oracle):
     w, _, _ = oracle.solve(mat, G, api.NONLOCAL, 8, ks[good], nthreads=4)
     assert np.abs(ev[good] - w).max() <= 1e-9
     assert np.all(np.isfinite(vec[good])) and np.all(np.isfinite(gr[good]))
+    assert np.all(np.isnan(vec[3])) and np.all(np.isnan(gr[3]))
     p.close()
 
 
