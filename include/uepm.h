@@ -158,6 +158,24 @@ int uepm_solve_device(uepm_plan* plan, const double* d_k, int64_t nk, double* d_
  * (Hamiltonian.cpp:130-133). */
 int uepm_plan_status(uepm_plan* plan, void* stream, int64_t* n_failed, int64_t* first_failed);
 
+/* Same as uepm_solve_device, without allocating.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * call makes no hipMalloc / hipFree and creates no stream or event: the chunk
+ * workspace is a UEPM_NOALLOC_SCRATCH_BYTES device array of the library that
+ * each GPU holds from its first uepm_ctx_create until the process ends.  The
+ * solve runs in chunks of the most k-points (at most 4096) whose workspace
+ * fits the array (uepm_plan_workspace_bytes(plan, 1) per k-point), one chunk at
+ * a time on `stream`; results are those of uepm_solve_device.  A plan that
+ * needs more than the array for one k-point -> UEPM_ERR_OUT_OF_RANGE.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads, other plans) race and must be ordered by the
+ * caller.  Stage timing (uepm_plan_set_timing) is not recorded.  Failures are
+ * read with uepm_plan_status as after uepm_solve_device. */
+#define UEPM_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int uepm_solve_device_noalloc(uepm_plan* plan, const double* d_k, int64_t nk, double* d_evals,
+                              double* d_evecs, double* d_grads, void* stream);
+
 /* Hamiltonian::SetMatrix + the hermitisation step of Diagonalize
  * (Hamiltonian.cpp:50-127): writes the full dim x dim complex128 matrix
  * 0.5*(H + H^dagger) for each k, column-major, interleaved (re,im).

@@ -78,6 +78,10 @@ def test_null_arguments_are_rejected():
     assert L.uepm_plan_create(None, C.byref(m), G.ctypes.data_as(C.POINTER(C.c_int32)), 1, 0, 1,
                               C.byref(plan)) == _abi.UEPM_ERR_INVALID_ARG
     assert "null" in L.uepm_last_error().decode()
+    k = np.zeros(3)
+    assert L.uepm_solve_device_noalloc(None, k.ctypes.data, 1, k.ctypes.data, None, None,
+                                       None) == _abi.UEPM_ERR_INVALID_ARG
+    assert "null plan" in L.uepm_last_error().decode()
     assert L.uepm_ctx_create(0, None) == _abi.UEPM_ERR_INVALID_ARG
     assert L.uepm_solve(None, None, 0, None, None, None) == _abi.UEPM_ERR_INVALID_ARG
 

@@ -111,6 +111,7 @@ def lib() -> C.CDLL:
     L.uepm_solve.argtypes = [P, D, I64, D, D, D]
     L.uepm_solve_device.argtypes = [P, P, I64, P, P, P, P]
     L.uepm_plan_status.argtypes = [P, P, C.POINTER(I64), C.POINTER(I64)]
+    L.uepm_solve_device_noalloc.argtypes = [P, P, I64, P, P, P, P]
     L.uepm_assemble_device.argtypes = [P, P, I64, P, P]
     L.uepm_plan_set_timing.argtypes = [P, C.c_int]
     L.uepm_plan_last_timing.argtypes = [P, C.c_int, D, C.POINTER(I64), C.POINTER(C.c_char_p)]

@@ -27,7 +27,7 @@ CXX = os.environ.get("CXX", "g++")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 DEVICE_SOURCES = ["solver_f64.hip", "tridiag_blocked.hip", "refine.hip", "copy.hip", "tridiag_reg.hip", "backtransform.hip",
-                  "tetra_dos.hip", "dielectric.hip"]
+                  "tetra_dos.hip", "dielectric.hip", "noalloc.hip"]
 HOST_SOURCES = ["uepm_abi.cpp", "runtime.cpp", "msh_io.cpp", "bzmesh.cpp", "dielectric.cpp"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", f"-I{CSRC}"]
 DEVICE_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

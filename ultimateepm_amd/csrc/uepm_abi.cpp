@@ -139,6 +139,34 @@ int plan_reserve(uepm_plan* p, int64_t chunk, int lanes) {
     return UEPM_OK;
 }
 
+size_t plan_carve(const uepm_plan* p, int64_t chunk, char* base, LaneView* v) {
+    const Sizes z = plan_sizes(p, chunk);
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void* {
+        if (!bytes) return nullptr;
+        void* ptr = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(size_t)255;
+        return ptr;
+    };
+    LaneView t;
+    t.H = static_cast<double2*>(take(z.H));
+    t.d = static_cast<double*>(take(z.d));
+    t.e = static_cast<double*>(take(z.e));
+    t.tau = static_cast<double2*>(take(z.tau));
+    t.Vf = static_cast<float2*>(take(z.Vf));
+    t.tauf = static_cast<float2*>(take(z.tauf));
+    t.S32 = static_cast<float2*>(take(z.S32));
+    t.Ev = static_cast<double*>(take(z.Ev));
+    t.X0 = static_cast<double2*>(take(z.X0));
+    t.S = static_cast<double2*>(take(z.S));
+    t.Z = static_cast<double*>(take(z.Z));
+    t.W = static_cast<double*>(take(z.W));
+    t.X = static_cast<double2*>(take(z.X));
+    t.kstat = static_cast<int*>(take(z.kstat));
+    if (base && v) *v = t;
+    return off;
+}
+
 void plan_apply_config(uepm_plan* p) {
     const bool mixed = p->flags & UEPM_MIXED;
     const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
@@ -147,6 +175,201 @@ void plan_apply_config(uepm_plan* p) {
     p->tp = uepm::tri_plan(p->dim, (int)sizeof(double2), p->ctx->lds_limit, bs);
     p->tp32 = uepm::tri_plan(p->dim, (int)sizeof(float2), p->ctx->lds_limit, bs);
     p->bt_bands = p->cfg.bt_bands > 0 ? p->cfg.bt_bands : 16;
+}
+
+}  // namespace uepm_host
+
+namespace uepm_host {
+
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    return hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
+}
+
+int solve_args_ok(const uepm_plan* p, int64_t nk, const double* dk, const double* devals, const double* devecs,
+                  const double* dgrads, const char* who) {
+    if (!p) return fail(UEPM_ERR_INVALID_ARG, std::string(who) + ": null plan");
+    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, std::string(who) + ": nk < 0");
+    if (nk == 0) return UEPM_OK;
+    if (!dk || !devals) return fail(UEPM_ERR_INVALID_ARG, std::string(who) + ": null k or eigenvalue buffer");
+    if (devecs && !(p->flags & UEPM_VECTORS))
+        return fail(UEPM_ERR_INVALID_ARG, "eigenvectors requested from a plan without UEPM_VECTORS");
+    if (dgrads && !(p->flags & UEPM_GRADIENTS))
+        return fail(UEPM_ERR_INVALID_ARG, "gradients requested from a plan without UEPM_GRADIENTS");
+    return UEPM_OK;
+}
+
+int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, double* devecs, double* dgrads,
+                 hipStream_t s, const LaneView* views, int lanes, int64_t chunk, bool no_alloc) {
+    const int n = p->dim, nb = p->nb;
+    const bool vec = p->flags & UEPM_VECTORS;
+    const bool need_vec = vec && (devecs || dgrads);
+    const bool mixed = p->flags & UEPM_MIXED;
+    const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
+    const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
+    const bool capturing = stream_capturing(s);
+    const int nchunks = (int)((nk + chunk - 1) / chunk);
+    // after the plan's previous solve, whichever stream it ran on (the
+    // workspace and the status words are the plan's); the no-allocation path
+    // creates no event, so it orders only after solves that created one
+    if (!capturing) {
+        if (p->lane_done[0]) HIP_OK(hipStreamWaitEvent(s, p->lane_done[0], 0));
+        else if (!no_alloc) HIP_OK(hipEventCreateWithFlags(&p->lane_done[0], hipEventDisableTiming));
+    }
+    unsigned long long* status = p->status.as<unsigned long long>();
+    HIP_OK(uepm::launch_status_reset(status, s));
+    if (lanes > 1) {
+        // the side stream starts after the caller's earlier work and the caller's
+        // stream resumes after the side stream's last chunk (join below)
+        if (!p->side) HIP_OK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+        if (!p->fork) HIP_OK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+        if (!p->join) HIP_OK(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
+        HIP_OK(hipEventRecord(p->fork, s));
+        HIP_OK(hipStreamWaitEvent(p->side, p->fork, 0));
+    }
+    const bool timing = p->timing && !no_alloc;
+    if (timing) {
+        const size_t need = (size_t)nchunks * kNumStages * 2;
+        while (p->ev.size() < need) {
+            hipEvent_t e;
+            HIP_OK(hipEventCreate(&e));
+            p->ev.push_back(e);
+        }
+    }
+    auto rec = [&](int chunk_i, int stage, int end) -> hipError_t {
+        if (!timing) return hipSuccess;
+        return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end],
+                              chunk_i % lanes ? p->side : s);
+    };
+    bool used[kNumStages] = {true, true, true, stage_vec, mixed};
+    p->stage_name[kTridiag] = mixed ? (p->tri_reg              ? "tridiag_reg_kernel<c64>"
+                                       : mixed_c64_tail(p) > 0 ? "tridiag_blocked+reg_kernel<c64>"
+                                                               : "tridiag_blocked_kernel<c64>")
+                                    : uepm::tridiag_reg_c128_tail(n) == n ? "tridiag_reg_kernel<c128>"
+                                    : uepm::tridiag_reg_c128_tail(n) > 0 ? "tridiag_blocked+reg_kernel<c128>"
+                                                                         : "tridiag_blocked_kernel<c128>";
+    p->stage_name[kTriEig] = "tri_eig_wave_kernel";
+    p->stage_name[kBack] = mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>";
+    p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
+    // the chunks; on an error the side stream is still joined into the caller's
+    auto run_chunks = [&]() -> int {
+        for (int ci = 0; ci < nchunks; ++ci) {
+            const int64_t c0 = (int64_t)ci * chunk;
+            const LaneView& w = views[ci % lanes];
+            const hipStream_t cs = ci % lanes ? p->side : s;
+            const int64_t c = std::min(chunk, nk - c0);
+            const double* kc = dk + 3 * c0;
+            double2* Hc = w.H;
+            int* ks = w.kstat;
+            HIP_OK(rec(ci, kAssemble, 0));
+            HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
+            HIP_OK(rec(ci, kAssemble, 1));
+            if (mixed) {
+                // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
+                // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
+                float2* Vc = w.Vf;
+                HIP_OK(rec(ci, kTridiag, 0));
+                if (p->tri_reg)
+                    HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d, w.e,
+                                                        w.tauf, w.S32, cs));
+                else {
+                    // n > 256: the blocked reduction for the leading steps, the register grids
+                    // for the trailing block it leaves in place in Vc
+                    const int t = mixed_c64_tail(p);
+                    const int s0 = n - t;
+                    HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, w.S32, Vc, w.d,
+                                                            w.e, w.tauf, cs,
+                                                            t > 0 ? s0 : n - 1));
+                    if (t > 0) {
+                        const int64_t nn = (int64_t)n * n;
+                        float2* B = Vc + s0 + (int64_t)s0 * n;
+                        HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, w.d + s0,
+                                                                  w.e + s0, w.tauf + s0, n,
+                                                                  w.S32, cs));
+                    }
+                }
+                HIP_OK(rec(ci, kTridiag, 1));
+                HIP_OK(rec(ci, kTriEig, 0));
+                // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
+                // for inverse iteration; the FP64 refinement sets the eigenvalues
+                HIP_OK(uepm::launch_tri_eig(w.d, w.e, n, nbr, c, w.Ev,
+                                            w.Z, w.W, kMixedInvSweeps, kMixedBisectTol, ks,
+                                            nullptr, c0, cs));
+                HIP_OK(rec(ci, kTriEig, 1));
+                HIP_OK(rec(ci, kBack, 0));
+                HIP_OK(uepm::launch_cluster_orth(w.d, w.e, w.Ev,
+                                                 w.Z, n, nbr, c, cs));
+                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf, w.Z, nbr,
+                                                w.X0, p->P, kc, p->G.as<int32_t>(), c, p->bt_bands, cs));
+                HIP_OK(rec(ci, kBack, 1));
+                HIP_OK(rec(ci, kRefine, 0));
+                HIP_OK(uepm::launch_refine(Hc, n, n, w.X0, nbr, nb, devals + c0 * nb,
+                                           devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
+                                           dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, ks,
+                                           status, c0, cs));
+                HIP_OK(rec(ci, kRefine, 1));
+                continue;
+            }
+            HIP_OK(rec(ci, kTridiag, 0));
+            {
+                // the blocked reduction for the leading steps, the FP64 register grids for the
+                // trailing block of size t (the whole matrix when n <= 192); reflectors in place in H
+                const int t = uepm::tridiag_reg_c128_tail(n);
+                const int s0 = n - t;
+                const int64_t nn = (int64_t)n * n;
+                if (t < n)
+                    HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, w.S, Hc, w.d,
+                                                            w.e, w.tau, cs,
+                                                            t > 0 ? s0 : n - 1));
+                if (t > 0) {
+                    double2* B = Hc + s0 + (int64_t)s0 * n;
+                    HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d + s0,
+                                                         w.e + s0, w.tau + s0, n,
+                                                         w.S, cs));
+                }
+            }
+            HIP_OK(rec(ci, kTridiag, 1));
+            HIP_OK(rec(ci, kTriEig, 0));
+            HIP_OK(uepm::launch_tri_eig(w.d, w.e, n, nb, c, devals + c0 * nb,
+                                        need_vec ? w.Z : nullptr, need_vec ? w.W : nullptr,
+                                        need_vec ? kFp64InvSweeps : 0, 0.0, need_vec ? ks : nullptr,
+                                        need_vec ? nullptr : status, c0, cs));
+            HIP_OK(rec(ci, kTriEig, 1));
+            if (need_vec) {
+                double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X;
+                HIP_OK(rec(ci, kBack, 0));
+                HIP_OK(uepm::launch_cluster_orth(w.d, w.e, devals + c0 * nb,
+                                                 w.Z, n, nb, c, cs));
+                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau, w.Z, nb, Xc,
+                                                dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(),
+                                                c, ks, status, c0, p->bt_bands, cs));
+                HIP_OK(rec(ci, kBack, 1));
+            }
+        }
+        return UEPM_OK;
+    };
+    const int rc = run_chunks();
+    if (lanes > 1) {
+        HIP_OK(hipEventRecord(p->join, p->side));
+        HIP_OK(hipStreamWaitEvent(s, p->join, 0));
+    }
+    if (!capturing && p->lane_done[0]) HIP_OK(hipEventRecord(p->lane_done[0], s));
+    if (rc != UEPM_OK) return rc;
+    if (timing) {
+        HIP_OK(hipStreamSynchronize(s));
+        for (int st = 0; st < kNumStages; ++st) {
+            p->stage_ms[st] = 0.0;
+            p->stage_launches[st] = used[st] ? nchunks : 0;
+            if (!used[st]) continue;
+            for (int ci = 0; ci < nchunks; ++ci) {
+                float ms = 0.f;
+                HIP_OK(hipEventElapsedTime(&ms, p->ev[((size_t)ci * kNumStages + st) * 2],
+                                           p->ev[((size_t)ci * kNumStages + st) * 2 + 1]));
+                p->stage_ms[st] += ms;
+            }
+        }
+    }
+    return UEPM_OK;
 }
 
 }  // namespace uepm_host
@@ -170,6 +393,13 @@ int uepm_ctx_create(int device, uepm_ctx** out) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         c->lds_limit = lds;
+    e = hipSetDevice(device);
+    const int st = e == hipSuccess ? noalloc_setup(device)
+                                   : fail(UEPM_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    if (st != UEPM_OK) {
+        delete c;
+        return st;
+    }
     *out = c;
     return UEPM_OK;
 }
@@ -557,193 +787,19 @@ int uepm_assemble_device(uepm_plan* p, const double* dk, int64_t nk, double* dH,
 
 int uepm_solve_device(uepm_plan* p, const double* dk, int64_t nk, double* devals, double* devecs, double* dgrads,
                       void* stream_) {
-    if (!p) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: null plan");
-    if (nk < 0) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: nk < 0");
-    if (nk == 0) return UEPM_OK;
-    if (!dk || !devals) return fail(UEPM_ERR_INVALID_ARG, "uepm_solve_device: null k or eigenvalue buffer");
-    const bool vec = p->flags & UEPM_VECTORS;
-    if (devecs && !vec) return fail(UEPM_ERR_INVALID_ARG, "eigenvectors requested from a plan without UEPM_VECTORS");
-    if (dgrads && !(p->flags & UEPM_GRADIENTS))
-        return fail(UEPM_ERR_INVALID_ARG, "gradients requested from a plan without UEPM_GRADIENTS");
+    const int v = solve_args_ok(p, nk, dk, devals, devecs, dgrads, "uepm_solve_device");
+    if (v != UEPM_OK || nk == 0) return v;
     HIP_OK(hipSetDevice(p->ctx->device));
     hipStream_t s = (hipStream_t)stream_;
-    const int n = p->dim, nb = p->nb;
-    const bool need_vec = vec && (devecs || dgrads);
-    const bool mixed = p->flags & UEPM_MIXED;
-    const int nbr = mixed ? std::min(n, nb + kMixedExtra) : nb;
-    const bool stage_vec = mixed || need_vec;  // eigenvectors of T are needed
     const int64_t chunk = plan_chunk(p, nk);
-    int lanes = plan_lanes(p, nk);
-    bool capturing = false;
-    {
-        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-        capturing = hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
-    }
-    if (capturing) lanes = 1;  // a stream under graph capture keeps one lane
+    const int lanes = stream_capturing(s) ? 1 : plan_lanes(p, nk);  // a stream under graph capture keeps one lane
     {
         const int st = plan_reserve(p, chunk, lanes);
         if (st != UEPM_OK) return st;
     }
-    const int nchunks = (int)((nk + chunk - 1) / chunk);
-    // after the plan's previous solve, whichever stream it ran on (the
-    // workspace and the status words are the plan's)
-    if (!capturing) {
-        if (!p->lane_done[0]) HIP_OK(hipEventCreateWithFlags(&p->lane_done[0], hipEventDisableTiming));
-        else HIP_OK(hipStreamWaitEvent(s, p->lane_done[0], 0));
-    }
-    unsigned long long* status = p->status.as<unsigned long long>();
-    HIP_OK(uepm::launch_status_reset(status, s));
-    if (lanes > 1) {
-        // the side stream starts after the caller's earlier work and the caller's
-        // stream resumes after the side stream's last chunk (join below)
-        if (!p->side) HIP_OK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-        if (!p->fork) HIP_OK(hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
-        if (!p->join) HIP_OK(hipEventCreateWithFlags(&p->join, hipEventDisableTiming));
-        HIP_OK(hipEventRecord(p->fork, s));
-        HIP_OK(hipStreamWaitEvent(p->side, p->fork, 0));
-    }
-    if (p->timing) {
-        const size_t need = (size_t)nchunks * kNumStages * 2;
-        while (p->ev.size() < need) {
-            hipEvent_t e;
-            HIP_OK(hipEventCreate(&e));
-            p->ev.push_back(e);
-        }
-    }
-    auto rec = [&](int chunk_i, int stage, int end) -> hipError_t {
-        if (!p->timing) return hipSuccess;
-        return hipEventRecord(p->ev[((size_t)chunk_i * kNumStages + stage) * 2 + end],
-                              chunk_i % lanes ? p->side : s);
-    };
-    bool used[kNumStages] = {true, true, true, stage_vec, mixed};
-    p->stage_name[kTridiag] = mixed ? (p->tri_reg              ? "tridiag_reg_kernel<c64>"
-                                       : mixed_c64_tail(p) > 0 ? "tridiag_blocked+reg_kernel<c64>"
-                                                               : "tridiag_blocked_kernel<c64>")
-                                    : uepm::tridiag_reg_c128_tail(n) == n ? "tridiag_reg_kernel<c128>"
-                                    : uepm::tridiag_reg_c128_tail(n) > 0 ? "tridiag_blocked+reg_kernel<c128>"
-                                                                         : "tridiag_blocked_kernel<c128>";
-    p->stage_name[kTriEig] = "tri_eig_wave_kernel";
-    p->stage_name[kBack] = mixed ? "cluster_orth+apply_q_kernel<c64>" : "cluster_orth+apply_q_kernel<c128>";
-    p->stage_name[kRefine] = uepm::refine_kernel_name(nbr);
-    // the chunks; on an error the side stream is still joined into the caller's
-    auto run_chunks = [&]() -> int {
-        for (int ci = 0; ci < nchunks; ++ci) {
-            const int64_t c0 = (int64_t)ci * chunk;
-            Lane& w = p->lane[ci % lanes];
-            const hipStream_t cs = ci % lanes ? p->side : s;
-            const int64_t c = std::min(chunk, nk - c0);
-            const double* kc = dk + 3 * c0;
-            double2* Hc = w.H.as<double2>();
-            int* ks = w.kstat.as<int>();
-            HIP_OK(rec(ci, kAssemble, 0));
-            HIP_OK(uepm::launch_assemble(p->P, p->G.as<int32_t>(), kc, c, Hc, n, cs));
-            HIP_OK(rec(ci, kAssemble, 1));
-            if (mixed) {
-                // precision form: complex64 reduction, FP64 tridiagonal eigen-stage,
-                // complex64 back-transform, FP64 Rayleigh-Ritz on the FP64 H(k)
-                float2* Vc = w.Vf.as<float2>();
-                HIP_OK(rec(ci, kTridiag, 0));
-                if (p->tri_reg)
-                    HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d.as<double>(), w.e.as<double>(),
-                                                        w.tauf.as<float2>(), w.S32.as<float2>(), cs));
-                else {
-                    // n > 256: the blocked reduction for the leading steps, the register grids
-                    // for the trailing block it leaves in place in Vc
-                    const int t = mixed_c64_tail(p);
-                    const int s0 = n - t;
-                    HIP_OK(uepm::launch_tridiag_blocked_c64(p->tp32, Hc, n, c, w.S32.p, Vc, w.d.as<double>(),
-                                                            w.e.as<double>(), w.tauf.as<float2>(), cs,
-                                                            t > 0 ? s0 : n - 1));
-                    if (t > 0) {
-                        const int64_t nn = (int64_t)n * n;
-                        float2* B = Vc + s0 + (int64_t)s0 * n;
-                        HIP_OK(uepm::launch_tridiag_reg_c64_block(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
-                                                                  w.e.as<double>() + s0, w.tauf.as<float2>() + s0, n,
-                                                                  w.S32.as<float2>(), cs));
-                    }
-                }
-                HIP_OK(rec(ci, kTridiag, 1));
-                HIP_OK(rec(ci, kTriEig, 0));
-                // T carries complex64 rounding (~1e-7 ||T||): brackets of 1e-10 ||T|| are ample
-                // for inverse iteration; the FP64 refinement sets the eigenvalues
-                HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nbr, c, w.Ev.as<double>(),
-                                            w.Z.as<double>(), w.W.as<double>(), kMixedInvSweeps, kMixedBisectTol, ks,
-                                            nullptr, c0, cs));
-                HIP_OK(rec(ci, kTriEig, 1));
-                HIP_OK(rec(ci, kBack, 0));
-                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), w.Ev.as<double>(),
-                                                 w.Z.as<double>(), n, nbr, c, cs));
-                HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf.as<float2>(), w.Z.as<double>(), nbr,
-                                                w.X0.as<double2>(), p->P, kc, p->G.as<int32_t>(), c, p->bt_bands, cs));
-                HIP_OK(rec(ci, kBack, 1));
-                HIP_OK(rec(ci, kRefine, 0));
-                HIP_OK(uepm::launch_refine(Hc, n, n, w.X0.as<double2>(), nbr, nb, devals + c0 * nb,
-                                           devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : nullptr,
-                                           dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(), c, ks,
-                                           status, c0, cs));
-                HIP_OK(rec(ci, kRefine, 1));
-                continue;
-            }
-            HIP_OK(rec(ci, kTridiag, 0));
-            {
-                // the blocked reduction for the leading steps, the FP64 register grids for the
-                // trailing block of size t (the whole matrix when n <= 192); reflectors in place in H
-                const int t = uepm::tridiag_reg_c128_tail(n);
-                const int s0 = n - t;
-                const int64_t nn = (int64_t)n * n;
-                if (t < n)
-                    HIP_OK(uepm::launch_tridiag_blocked_f64(p->tp, Hc, n, c, w.S.p, Hc, w.d.as<double>(),
-                                                            w.e.as<double>(), w.tau.as<double2>(), cs,
-                                                            t > 0 ? s0 : n - 1));
-                if (t > 0) {
-                    double2* B = Hc + s0 + (int64_t)s0 * n;
-                    HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d.as<double>() + s0,
-                                                         w.e.as<double>() + s0, w.tau.as<double2>() + s0, n,
-                                                         w.S.as<double2>(), cs));
-                }
-            }
-            HIP_OK(rec(ci, kTridiag, 1));
-            HIP_OK(rec(ci, kTriEig, 0));
-            HIP_OK(uepm::launch_tri_eig(w.d.as<double>(), w.e.as<double>(), n, nb, c, devals + c0 * nb,
-                                        need_vec ? w.Z.as<double>() : nullptr, need_vec ? w.W.as<double>() : nullptr,
-                                        need_vec ? kFp64InvSweeps : 0, 0.0, need_vec ? ks : nullptr,
-                                        need_vec ? nullptr : status, c0, cs));
-            HIP_OK(rec(ci, kTriEig, 1));
-            if (need_vec) {
-                double2* Xc = devecs ? reinterpret_cast<double2*>(devecs) + (size_t)c0 * nb * n : w.X.as<double2>();
-                HIP_OK(rec(ci, kBack, 0));
-                HIP_OK(uepm::launch_cluster_orth(w.d.as<double>(), w.e.as<double>(), devals + c0 * nb,
-                                                 w.Z.as<double>(), n, nb, c, cs));
-                HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau.as<double2>(), w.Z.as<double>(), nb, Xc,
-                                                dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(),
-                                                c, ks, status, c0, p->bt_bands, cs));
-                HIP_OK(rec(ci, kBack, 1));
-            }
-        }
-        return UEPM_OK;
-    };
-    const int rc = run_chunks();
-    if (lanes > 1) {
-        HIP_OK(hipEventRecord(p->join, p->side));
-        HIP_OK(hipStreamWaitEvent(s, p->join, 0));
-    }
-    if (!capturing) HIP_OK(hipEventRecord(p->lane_done[0], s));
-    if (rc != UEPM_OK) return rc;
-    if (p->timing) {
-        HIP_OK(hipStreamSynchronize(s));
-        for (int st = 0; st < kNumStages; ++st) {
-            p->stage_ms[st] = 0.0;
-            p->stage_launches[st] = used[st] ? nchunks : 0;
-            if (!used[st]) continue;
-            for (int ci = 0; ci < nchunks; ++ci) {
-                float ms = 0.f;
-                HIP_OK(hipEventElapsedTime(&ms, p->ev[((size_t)ci * kNumStages + st) * 2],
-                                           p->ev[((size_t)ci * kNumStages + st) * 2 + 1]));
-                p->stage_ms[st] += ms;
-            }
-        }
-    }
-    return UEPM_OK;
+    LaneView views[2];
+    for (int l = 0; l < lanes; ++l) views[l] = lane_view(p->lane[l]);
+    return solve_chunks(p, dk, nk, devals, devecs, dgrads, s, views, lanes, chunk, false);
 }
 
 int uepm_device_alloc(uepm_ctx* ctx, uint64_t bytes, void** ptr) {

@@ -118,6 +118,62 @@ struct uepm_plan {
 };
 
 namespace uepm_host {
+// One lane's workspace as raw device pointers: the plan's own buffers
+// (lane_view), or a region carved out of a caller-independent scratch array
+// (plan_carve).
+struct LaneView {
+    double2* H = nullptr;
+    double* d = nullptr;
+    double* e = nullptr;
+    double2* tau = nullptr;
+    float2* Vf = nullptr;
+    float2* tauf = nullptr;
+    float2* S32 = nullptr;
+    double* Ev = nullptr;
+    double2* X0 = nullptr;
+    double2* S = nullptr;
+    double* Z = nullptr;
+    double* W = nullptr;
+    double2* X = nullptr;
+    int* kstat = nullptr;
+};
+inline LaneView lane_view(const Lane& w) {
+    LaneView v;
+    v.H = w.H.as<double2>();
+    v.d = w.d.as<double>();
+    v.e = w.e.as<double>();
+    v.tau = w.tau.as<double2>();
+    v.Vf = w.Vf.as<float2>();
+    v.tauf = w.tauf.as<float2>();
+    v.S32 = w.S32.as<float2>();
+    v.Ev = w.Ev.as<double>();
+    v.X0 = w.X0.as<double2>();
+    v.S = w.S.as<double2>();
+    v.Z = w.Z.as<double>();
+    v.W = w.W.as<double>();
+    v.X = w.X.as<double2>();
+    v.kstat = w.kstat.as<int>();
+    return v;
+}
+// Bytes one lane of `chunk` k-points takes when its buffers are laid out
+// back to back at 256-byte boundaries from `base`; with base != nullptr the
+// view's pointers are set to that layout.
+size_t plan_carve(const uepm_plan* p, int64_t chunk, char* base, LaneView* view);
+// hipStreamIsCapturing reports a capture in progress (or fails)
+bool stream_capturing(hipStream_t s);
+// argument checks shared by the device-buffer solve entry points
+int solve_args_ok(const uepm_plan* p, int64_t nk, const double* dk, const double* devals, const double* devecs,
+                  const double* dgrads, const char* who);
+// The solve of nk k-points in chunks of `chunk` over the lane workspaces
+// views[0..lanes): status reset, ordering after the plan's previous solve,
+// the stages of every chunk (alternate chunks on the plan's side stream when
+// lanes == 2), stage timing.  no_alloc: one lane, and no event, stream or
+// device memory is created (stage timing is then not recorded).
+int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, double* devecs, double* dgrads,
+                 hipStream_t s, const LaneView* views, int lanes, int64_t chunk, bool no_alloc);
+// Looks up the no-allocation scratch array's address on `device` (current),
+// once per device; called by uepm_ctx_create.
+int noalloc_setup(int device);
 // k-points per launch for a solve of nk points under the plan's configuration
 int64_t plan_chunk(const uepm_plan* p, int64_t nk);
 // lanes a solve of nk points runs (2 when it spans two or more chunks, unless
