@@ -24,8 +24,19 @@ if ROOT not in sys.path:
 METRIC = "k-pts/s Si nonlocal N≈181 FP64 vs mixed @≤0.1 meV; serving p50/p99 latency"
 FP64_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
+FP32_VALU_PEAK_TFLOPS = 78.6  # non-packed FP32 FMA on the vector ALUs (packed FP32 is not used)
+# which datapath each reduction kernel issues its arithmetic on
+DATAPATH = {
+    ("mixed", "tridiag_reg_kernel"): "VALU, non-packed FP32 (78.6 TF/s): no MFMA in the register-grid chain",
+    ("fp64", "tridiag_reg_kernel"): "VALU FP64 (78.6 TF/s, equal to the FP64 MFMA rate)",
+}
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
+
+
+def log(msg: str) -> None:
+    """A progress line on stderr (the JSON result is the one stdout line)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
 def tridiag_flops(n: int) -> float:
@@ -33,26 +44,54 @@ def tridiag_flops(n: int) -> float:
     return 16.0 / 3.0 * n ** 3
 
 
+def host_cpu() -> dict:
+    """CPU model and the cores this process may use (the job's share of the host)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = min(affinity, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else affinity
+    return {"model": model, "host_cpus": os.cpu_count(), "affinity_cpus": affinity, "threads": max(1, threads)}
+
+
 def cpu_baseline(mat, G, nb, ks, seconds: float):
-    """The oracle (C++ restatement of Compute_parallel, OpenMP) on a bounded sample."""
+    """The oracle (C++ restatement of Compute_parallel, OpenMP, one solver object per
+    thread, dynamic schedule) on a bounded sample of the headline k-set: on every
+    core of the job's CPU share and on one core, with the assembly (Bessel-heavy
+    non-local term) vs eigensolve split of the one-core run."""
     from oracle import oracle as O
     from ultimateepm_amd import api
 
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    cpu = host_cpu()
+    threads = cpu["threads"]
     flags = api.NONLOCAL | api.VECTORS
     probe = ks[: 2 * threads]
     t0 = time.perf_counter()
     O.solve(mat, G, flags, nb, probe, nthreads=threads)
     rate = len(probe) / (time.perf_counter() - t0)
     m = int(max(2 * threads, min(len(ks), rate * seconds)))
-    sample = ks[:m]
     t0 = time.perf_counter()
-    O.solve(mat, G, flags, nb, sample, nthreads=threads)
+    O.solve(mat, G, flags, nb, ks[:m], nthreads=threads)
     dt = time.perf_counter() - t0
+    # one core: a quarter of the time budget
+    m1 = int(max(4, min(len(ks), rate / threads * seconds / 4)))
+    t0 = time.perf_counter()
+    O.solve(mat, G, flags, nb, ks[:m1], nthreads=1)
+    dt1 = time.perf_counter() - t0
+    # assembly alone (SetMatrix + hermitisation) on the same one-core sample
+    t0 = time.perf_counter()
+    for k in ks[:m1]:
+        O.assemble(mat, G, k, nl=True, soc=False, herm=True)
+    dt_asm = time.perf_counter() - t0
     return {
         "value": round(m / dt, 2),
         "unit": "k-pts/s",
@@ -61,6 +100,15 @@ def cpu_baseline(mat, G, nb, ks, seconds: float):
         "sample": f"first {m} k-points of the same seeded BZ set, Si non-local N={len(G)}, {nb} bands + "
                   f"eigenvectors (all eigenvectors, as Eigen computes them), oracle built -O3 "
                   f"-march=x86-64-v3 -ffp-contract=off, {threads} OpenMP threads, {dt:.1f} s",
+        "cpu_model": cpu["model"],
+        "host_cpus": cpu["host_cpus"],
+        "affinity_cpus": cpu["affinity_cpus"],
+        "one_core_kpts_per_s": round(m1 / dt1, 3),
+        "one_core_sample": f"first {m1} k-points, 1 thread, {dt1:.1f} s",
+        "one_core_split": {"assembly_ms_per_k": round(dt_asm / m1 * 1e3, 2),
+                           "eigensolve_ms_per_k": round(max(dt1 - dt_asm, 0.0) / m1 * 1e3, 2),
+                           "assembly_frac": round(dt_asm / dt1, 3)},
+        "parallel_speedup": round((m / dt) / (m1 / dt1), 2),
     }
 
 
@@ -105,10 +153,14 @@ def copy_calibration(mib: int, reps: int) -> int:
 
 
 def serving_leg(ctx, requests: int, seed: int = SEED, budget_gib: int = 64, inflight: int = 8) -> dict:
-    """Serving form: Poisson arrivals at 200 req/s (seeded); material uniform over
-    Si/Ge/GaAs/InP, n in 6..14 shells, SOC on for half (off for InP), log-uniform
-    1..4096 k-points, 8..24 bands; the same request stream in FP64 and in the
-    precision form on a continuous-batching server (HBM budget, in-flight slots)."""
+    """Serving form: Poisson arrivals (seeded); material uniform over Si/Ge/GaAs/InP,
+    n in 6..14 shells, SOC on for half (off for InP), log-uniform 1..4096
+    k-points, 8..24 bands; the same request stream in FP64 and in the precision
+    form on a continuous-batching server (HBM budget, in-flight slots).  Three
+    operating points per form: the configuration's 200 req/s; saturation (every
+    request submitted at once: the throughput the server sustains); and a
+    Poisson stream at half the saturation request rate, where latency is
+    service time plus modest queueing."""
     from ultimateepm_amd import api, serving
 
     wl = serving.poisson_workload(seed, requests, rate=200.0, n_min=6, n_max=14, k_max=4096)
@@ -118,13 +170,66 @@ def serving_leg(ctx, requests: int, seed: int = SEED, budget_gib: int = 64, infl
                        "1..4096; nb 8..24; non-local" % seed}
     res = {}
     for mode in ("fp64", "mixed"):
-        serving.run(srv, wl[: min(8, len(wl))], mixed=(mode == "mixed"), realtime=False)  # plan warm-up
-        r, summ = serving.run(srv, wl, mixed=(mode == "mixed"), realtime=True)
+        mixed = mode == "mixed"
+        serving.run(srv, wl[: min(8, len(wl))], mixed=mixed, realtime=False)  # plan warm-up
+        r, summ = serving.run(srv, wl, mixed=mixed, realtime=True)
         res[mode] = r
         out[mode] = summ
+        log(f"serving {mode}: 200 req/s done, saturation run")
+        _, sat = serving.run(srv, wl, mixed=mixed, realtime=False)
+        out[f"{mode}_saturation"] = sat
+        cap_req_s = sat["requests"] / sat["elapsed_s"]
+        rate = 0.5 * cap_req_s
+        wl_half = serving.poisson_workload(seed + 1, requests, rate=rate, n_min=6, n_max=14, k_max=4096)
+        log(f"serving {mode}: half-load run at {rate:.1f} req/s")
+        _, half = serving.run(srv, wl_half, mixed=mixed, realtime=True)
+        half["offered_rate_req_s"] = round(rate, 2)
+        out[f"{mode}_half_load"] = half
     worst = max(float(np.abs(a[0] - b[0]).max()) for a, b in zip(res["fp64"], res["mixed"]))
     out["mixed_vs_fp64_max_meV"] = worst * 1e3
     srv.close()
+    return out
+
+
+def config3_leg(ctx, nk: int = 100_000, seed: int = SEED) -> dict:
+    """Precision form at the spinor size: Ge with spin-orbit (materials.yaml,
+    non-local on), n = 10 shells (dim 2 x 137 = 274), nk seeded first-BZ
+    k-points, lowest 16 eigenvalues; both forms, one untimed and one timed
+    solve each (device-resident k), max / RMS difference in meV."""
+    from ultimateepm_amd import api
+    from ultimateepm_amd.material import load
+
+    mat = load("Ge")
+    G = api.basis_vectors(10)
+    nb = 16
+    ks = api.bz_sample(seed + 3, nk)
+    dk = api.DeviceArray.from_host(ctx, ks)
+    out = {"workload": f"Ge + SOC (materials.yaml, non-local), n=10 (dim {2 * len(G)}), {nk} seeded first-BZ "
+                       f"k-points, lowest {nb} eigenvalues", "nk": nk, "dim": 2 * len(G), "nb": nb}
+    ev = {}
+    for mode in ("fp64", "mixed"):
+        flags = api.NONLOCAL | api.SOC | (api.MIXED if mode == "mixed" else 0)
+        p = api.Plan(ctx, mat, G, flags, nb)
+        de = api.DeviceArray(ctx, (nk, nb))
+        p.solve_device(dk.ptr, nk, de.ptr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        p.solve_device(dk.ptr, nk, de.ptr)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        p.set_timing(True)
+        p.solve_device(dk.ptr, nk, de.ptr)
+        ctx.synchronize()
+        out[f"{mode}_kpts_per_s"] = round(nk / dt, 1)
+        out[f"{mode}_stage_ms"] = {k: round(v[0], 3) for k, v in p.last_timing().items()}
+        ev[mode] = de.numpy()
+        p.close()
+        de.free()
+    d = ev["mixed"] - ev["fp64"]
+    out["mixed_vs_fp64_max_meV"] = float(np.abs(d).max() * 1e3)
+    out["mixed_vs_fp64_rms_meV"] = float(np.sqrt(np.mean(d ** 2)) * 1e3)
+    out["mixed_over_fp64"] = round(out["mixed_kpts_per_s"] / out["fp64_kpts_per_s"], 3)
+    dk.free()
     return out
 
 
@@ -208,6 +313,7 @@ def batching_leg(ctx, nk_per_item: int, branches: int = 8, ws_cap_gib: float = 2
             bufs.append((dk, de))
             items.append((p, dk.ptr, nk_per_item, de.ptr))
         t_tune = time.perf_counter() - t_tune
+        log(f"batching {mode}: tuned and set up in {t_tune:.1f} s, capturing the graph")
         b = api.Batch(ctx, items, branches=branches)
         b.launch()
         ctx.synchronize()
@@ -453,6 +559,9 @@ def main() -> int:
     ap.add_argument("--nb", type=int, default=12)
     ap.add_argument("--shells", type=int, default=12)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--timing-steps", type=int, default=2, help="extra steps with per-stage HIP-event timing")
+    ap.add_argument("--config3-nk", type=int, default=100_000,
+                    help="k-points of the Ge + SOC (dim 274) leg of the default run (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
     ap.add_argument("--lanes", type=int, default=0, choices=[0, 1, 2],
@@ -462,6 +571,8 @@ def main() -> int:
                          "alone; dims: throughput by basis dimension")
     ap.add_argument("--requests", type=int, default=200, help="requests of --form serving")
     ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
+    ap.add_argument("--batch-leg-nk", type=int, default=20000,
+                    help="k-points per plan of the batching leg of the default run (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=100,
                     help="requests of the serving leg of the default run (0: skip)")
     ap.add_argument("--mesh-steps", type=int, default=3,
@@ -473,6 +584,7 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the copy kernel over MIB MiB (profiler counter calibration)")
     args = ap.parse_args()
+    args.timing_steps = max(1, args.timing_steps)
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration, 10)
 
@@ -546,40 +658,54 @@ def main() -> int:
         for _ in range(args.warmup):
             step()
         stream.synchronize()
-        plan.set_timing(True)
-        stage_ms = {}
         ctx.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(args.steps):  # stage timing off: no host synchronisation inside a step
+            step()
+        stream.synchronize()
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t0
+        evals = dev.numpy()
+        # the stage split, from HIP events around each stage launch, in separate steps
+        plan.set_timing(True)
+        stage_ms = {}
+        for _ in range(args.timing_steps):
             step()
             for name, (ms, cnt) in plan.last_timing().items():
                 acc = stage_ms.setdefault(name, [0.0, 0])
                 acc[0] += ms
                 acc[1] += cnt
-        stream.synchronize()
-        ctx.synchronize()
-        elapsed = time.perf_counter() - t0
-        evals = dev.numpy()
+        plan.set_timing(False)
         plan.close()
         dev.free()
         dvec.free()
-        return {"elapsed": elapsed, "stage_ms": stage_ms, "evals": evals, "dim": plan.dim, "lanes": lanes}
+        return {"elapsed": elapsed, "stage_ms": stage_ms, "evals": evals, "dim": plan.dim, "lanes": lanes,
+                "timing_steps": args.timing_steps}
 
-    res = {m: run(m) for m in args.modes.split(",")}
+    res = {}
+    for m in args.modes.split(","):
+        log(f"headline {m}: {args.warmup} warm-up + {args.steps} timed steps of {args.nk} k-points")
+        res[m] = run(m)
+        log(f"headline {m}: {args.nk / (res[m]['elapsed'] / args.steps):.0f} k-pts/s")
     head = "mixed" if "mixed" in res else "fp64"
     r = res[head]
     ms_per_step = r["elapsed"] / args.steps * 1e3
     value = args.nk / (r["elapsed"] / args.steps)
     stage_ms = r["stage_ms"]
+    ts = r["timing_steps"]
     dominant = max(stage_ms, key=lambda k: stage_ms[k][0])
     dom_ms, dom_launches = stage_ms[dominant]
     if dominant.startswith("tridiag"):
-        flops = tridiag_flops(r["dim"]) * args.nk * args.steps
+        flops = tridiag_flops(r["dim"]) * args.nk * ts
         peak = FP32_MFMA_PEAK_TFLOPS if head == "mixed" else FP64_PEAK_TFLOPS
     else:
         flops = 0.0
         peak = FP64_PEAK_TFLOPS
     achieved = flops / (dom_ms * 1e-3) / 1e12
+    # the reduction's flops of a whole step over that step's wall time (timing off):
+    # the chip-level rate with both lanes in flight and every other stage included
+    step_s = r["elapsed"] / args.steps
+    chip = tridiag_flops(r["dim"]) * args.nk / step_s / 1e12
     roofline = {
         "kernel": dominant,
         "bound": "mfma",
@@ -589,17 +715,14 @@ def main() -> int:
         "frac": round(achieved / peak, 4),
         "traffic": None,
         "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
-        "k_per_launch": round(args.nk * args.steps / max(dom_launches, 1), 1),
+        "k_per_launch": round(args.nk * ts / max(dom_launches, 1), 1),
         "flop_per_k": tridiag_flops(r["dim"]),
-        "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in stage_ms.items()},
-        # Two lanes run alternate chunks through the same stage sequence side by
-        # side, so each launch shares the chip with the other lane's launch of the
-        # same kernel: the chip-level rate is lanes x the per-launch rate (check:
-        # the stage sum over lanes is about ms_per_step).
+        "datapath": DATAPATH.get((head, dominant.split("<")[0]), "see DESIGN.md"),
+        "stage_ms_per_step": {k: round(v[0] / ts, 3) for k, v in stage_ms.items()},
         "launches_in_flight": r["lanes"],
-        "chip_achieved": round(achieved * r["lanes"], 3),
-        "chip_frac": round(achieved * r["lanes"] / peak, 4),
-        "stage_sum_over_lanes_ms": round(sum(v[0] for v in stage_ms.values()) / args.steps / r["lanes"], 3),
+        "chip_reduction_TFLOPs_over_step": round(chip, 3),
+        "chip_frac_over_step": round(chip / peak, 4),
+        "stage_sum_over_lanes_ms": round(sum(v[0] for v in stage_ms.values()) / ts / r["lanes"], 3),
     }
     bpk = measured_traffic(head, dominant, N)
     if bpk is not None:
@@ -608,7 +731,7 @@ def main() -> int:
     extra = {}
     for m, rr in res.items():
         extra[f"{m}_kpts_per_s"] = round(args.nk / (rr["elapsed"] / args.steps), 1)
-        extra[f"{m}_stage_ms_per_step"] = {k: round(v[0] / args.steps, 3) for k, v in rr["stage_ms"].items()}
+        extra[f"{m}_stage_ms_per_step"] = {k: round(v[0] / rr["timing_steps"], 3) for k, v in rr["stage_ms"].items()}
     if "mixed" in res and "fp64" in res:
         d = res["mixed"]["evals"] - res["fp64"]["evals"]
         extra["mixed_vs_fp64_max_meV"] = float(np.abs(d).max() * 1e3)
@@ -639,7 +762,14 @@ def main() -> int:
         },
         "roofline": roofline,
     }
+    if args.config3_nk > 0:
+        log("Ge + SOC (dim 274) leg")
+        out["config3_ge_soc"] = config3_leg(ctx, args.config3_nk)
+    if args.batch_leg_nk > 0:
+        log("batching leg (28 plans, tuned, one graph launch per form)")
+        out["batching"] = batching_leg(ctx, args.batch_leg_nk)
     if args.serving_requests > 0:
+        log("serving leg")
         sv = serving_leg(ctx, args.serving_requests)
         out["serving"] = {
             "workload": sv["workload"], "requests": sv["requests"],
@@ -648,13 +778,24 @@ def main() -> int:
             "mixed_kpts_per_s": sv["mixed"]["kpts_per_s"], "fp64_kpts_per_s": sv["fp64"]["kpts_per_s"],
             "mixed_vs_fp64_max_meV": sv["mixed_vs_fp64_max_meV"],
             "hbm_budget_gib": sv["hbm_budget_gib"], "max_inflight": sv["max_inflight"]}
+        for mode in ("mixed", "fp64"):
+            sat, half = sv[f"{mode}_saturation"], sv[f"{mode}_half_load"]
+            out["serving"][f"{mode}_saturation_kpts_per_s"] = sat["kpts_per_s"]
+            out["serving"][f"{mode}_saturation_req_per_s"] = round(sat["requests"] / sat["elapsed_s"], 2)
+            out["serving"][f"{mode}_half_load_req_per_s"] = half["offered_rate_req_s"]
+            out["serving"][f"{mode}_half_load_p50_ms"] = half["latency_p50_ms"]
+            out["serving"][f"{mode}_half_load_p99_ms"] = half["latency_p99_ms"]
     if args.mesh_steps > 0:
+        log("mesh leg")
         out["mesh"] = mesh_leg(ctx, args.mesh_steps, cpu=not args.no_cpu_baseline)
     if args.dielectric_q > 0:
+        log("dielectric leg")
         out["dielectric"] = dielectric_leg(ctx, args.dielectric_q, cpu=not args.no_cpu_baseline)
     if args.states_steps > 0:
+        log("BZ_States leg")
         out["bz_states"] = states_leg(ctx, args.states_steps, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
+        log("CPU baseline (oracle, all cores of the job's share, then one core)")
         out["cpu_baseline"] = cpu_baseline(mat, G, nb, ks, args.cpu_seconds)
     print(json.dumps(out), flush=True)
     stream.close()
