@@ -101,6 +101,8 @@ struct Cursor {
         v = x;
         return true;
     }
+    // the most entries of at least `min_chars` characters that the rest of the input can hold
+    uint64_t room(uint64_t min_chars) const { return static_cast<uint64_t>(end - p) / min_chars + 1; }
     void to_line_end() {
         while (p < end && *p != '\n') ++p;
         if (p < end) ++p;
@@ -130,12 +132,15 @@ int read_nodes(Cursor& c, uepm_msh& m, const std::string& path) {
     if (!c.u64(nblocks) || !c.u64(nnodes) || !c.u64(mintag) || !c.u64(maxtag))
         return parse_error(path, "bad $Nodes header");
     const size_t base = m.tags.size();
-    m.tags.reserve(base + nnodes);
-    m.xyz.reserve(3 * (base + nnodes));
+    // a count read from the file reserves no more than the rest of the file can hold
+    const uint64_t room = c.room(2);
+    m.tags.reserve(base + std::min(nnodes, room));
+    m.xyz.reserve(3 * (base + std::min(nnodes, room)));
     for (uint64_t b = 0; b < nblocks; ++b) {
         int64_t dim, etag, param;
         uint64_t n;
         if (!c.i64(dim) || !c.i64(etag) || !c.i64(param) || !c.u64(n)) return parse_error(path, "bad node block header");
+        if (dim < 0 || dim > 3) return parse_error(path, "bad node block dimension");
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t t;
             if (!c.u64(t)) return parse_error(path, "truncated node tags");
@@ -153,6 +158,8 @@ int read_nodes(Cursor& c, uepm_msh& m, const std::string& path) {
         }
     }
     if (m.tags.size() - base != nnodes) return parse_error(path, "node count differs from the $Nodes header");
+    if (m.tags.size() > static_cast<size_t>(std::numeric_limits<int32_t>::max()))
+        return parse_error(path, "more nodes than 32-bit vertex indices address");
     return skip_section(c, "Nodes") ? UEPM_OK : parse_error(path, "missing $EndNodes");
 }
 
@@ -192,7 +199,7 @@ int read_elements(Cursor& c, uepm_msh& m, const std::string& path) {
         if (type != 4)
             return fail(UEPM_ERR_INVALID_ARG, "uepm_msh_read: " + path + ": volume element type " +
                                                   std::to_string(type) + " (only 4-node tetrahedra are supported)");
-        m.tet.reserve(m.tet.size() + 4 * n);
+        m.tet.reserve(m.tet.size() + 4 * std::min(n, c.room(10)));
         for (uint64_t i = 0; i < n; ++i) {
             uint64_t et, v[4];
             if (!c.u64(et) || !c.u64(v[0]) || !c.u64(v[1]) || !c.u64(v[2]) || !c.u64(v[3]))

@@ -2,6 +2,7 @@
 // NonLocalParameters.hpp:91-116, SpinOrbitParameters.hpp:80-87) and the C-ABI view.
 #include "Material.h"
 
+#include <cstdint>
 #include <stdexcept>
 
 #include "yaml_reader.h"
@@ -17,6 +18,15 @@ double req(const yaml::Node* n, const char* key) {
     const yaml::Node* v = n ? n->get(key) : nullptr;
     if (!v) throw std::runtime_error(std::string("Materials::load_material_parameters: missing key ") + key);
     return v->as_double(key);
+}
+
+// an unsigned integer value (the reference converts these with as<unsigned int>, which refuses other values)
+unsigned req_unsigned(const yaml::Node* n, const char* key) {
+    const double x = req(n, key);
+    if (!(x >= 0.0 && x <= 4294967295.0 && x == static_cast<double>(static_cast<uint64_t>(x))))
+        throw std::runtime_error(std::string("Materials::load_material_parameters: ") + key +
+                                 " is not an unsigned integer");
+    return static_cast<unsigned>(x);
 }
 }  // namespace
 
@@ -63,8 +73,8 @@ void Materials::load_material_parameters(const std::string& filename) {
         }
         if (const yaml::Node* so = m->get("spin-orbit-parameters")) {
             SpinOrbitParameters p;
-            p.m_period_anion = static_cast<unsigned>(req(so, "period_anion"));
-            p.m_period_cation = static_cast<unsigned>(req(so, "period_cation"));
+            p.m_period_anion = req_unsigned(so, "period_anion");
+            p.m_period_cation = req_unsigned(so, "period_cation");
             p.m_radial_extent_anion = req(so, "radial_extent_anion");
             p.m_radial_extent_cation = req(so, "radial_extent_cation");
             p.m_alpha = req(so, "alpha_soc");
