@@ -322,10 +322,25 @@ def batching_leg(ctx, nk_per_item: int, branches: int = 8, ws_cap_gib: float = 2
         ctx.synchronize()
         dt = time.perf_counter() - t0
         evals[mode] = [de.numpy() for _, de in bufs]
+        b.close()
+        # the same plans solved one after another, each alone on the chip (one
+        # untimed solve first): what the graph's concurrent branches save, and
+        # where the time goes by dimension
+        alone, by_dim = 0.0, {}
+        for p, (dk, de) in zip(plans, bufs):
+            p.solve_device(dk.ptr, nk_per_item, de.ptr)
+            ctx.synchronize()
+            t1 = time.perf_counter()
+            p.solve_device(dk.ptr, nk_per_item, de.ptr)
+            ctx.synchronize()
+            t1 = time.perf_counter() - t1
+            alone += t1
+            by_dim[p.dim] = by_dim.get(p.dim, 0.0) + t1
         out[mode] = {"kpts_per_s": round(len(combos) * nk_per_item / dt, 1), "graph_launch_s": round(dt, 4),
+                     "plans_alone_s": round(alone, 4), "graph_over_alone": round(dt / alone, 3),
+                     "alone_s_by_dim": {f"dim{d}": round(t, 4) for d, t in sorted(by_dim.items())},
                      "tune_and_setup_s": round(t_tune, 2),
                      "tuned": {f"dim{d}": c for (d, _), c in sorted(tuned.items())}}
-        b.close()
         for p in plans:
             p.close()
         for dk, de in bufs:
