@@ -198,10 +198,10 @@ int uepm_plan_last_timing(const uepm_plan* plan, int stage, double* ms, int64_t*
 /* ---- configuration, tuning and workspace -----------------------------------
  * No counterpart in the reference (its solve has no tunables). */
 typedef struct uepm_config {
-    int32_t tridiag;   /* 0 automatic (UEPM_MIXED plans above dim 256: LDS-blocked
+    int32_t tridiag;   /* 0 automatic (UEPM_MIXED plans above dim 288: LDS-blocked
                           leading steps, then the register grids); 1 LDS-blocked
                           reduction only; 2 register-resident complex64 reduction
-                          (UEPM_MIXED plans, dim <= 256)                              */
+                          (UEPM_MIXED plans, dim <= 288)                              */
     int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = 512)  */
     int32_t bt_bands;  /* bands per back-transform workgroup: 4, 8 or 16 (0 = 16)    */
     int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */

@@ -199,7 +199,7 @@ def test_mixed_precision_vs_oracle(uepm, oracle, gpu_ctx, sym, n, soc, nb):
 def test_mixed_precision_wide_register_grids(uepm, oracle, gpu_ctx, ng):
     """Precision form on the first ng vectors of the n = 14 basis: dims 200
     and 240 start on the 224- and 256-wide register grids, 256 fills the
-    widest, and 259 (the whole basis) hands over from the blocked reduction."""
+    256-wide one, and 259 (the whole basis) starts on the 288-wide grid."""
     print(f"case: mixed Si non-local, first {ng} vectors of the n=14 basis", flush=True)
     mat = _mat("Si")
     G = np.ascontiguousarray(uepm.basis_vectors(14)[:ng])
@@ -214,6 +214,29 @@ def test_mixed_precision_wide_register_grids(uepm, oracle, gpu_ctx, ng):
     X = vec[2].T
     assert np.abs(X.conj().T @ X - np.eye(12)).max() <= 1e-10
     assert np.linalg.norm(H @ X - X * ev[2], axis=0).max() <= 1e-3
+    plan.close()
+
+
+@pytest.mark.parametrize("ng", [140, 144, 145])
+def test_mixed_precision_288_grid_boundary(uepm, oracle, gpu_ctx, ng):
+    """Precision form with spin-orbit on the first ng vectors of the n = 11
+    basis (Ge): dims 280 and 288 run the whole reduction on the register grids
+    from the 288-wide one; dim 290 starts in the blocked reduction, which
+    hands a trailing block of 286 to the 288-wide grid."""
+    print(f"case: mixed Ge SOC, first {ng} vectors of the n=11 basis (dim {2 * ng})", flush=True)
+    mat = _mat("Ge")
+    G = np.ascontiguousarray(uepm.basis_vectors(11)[:ng])
+    plan = uepm.Plan(gpu_ctx, mat, G, uepm.NONLOCAL | uepm.SOC | uepm.MIXED | uepm.VECTORS, 16)
+    ks = np.vstack([[0.0, 0.0, 0.0], uepm.bz_sample(91, 6)])
+    ev, vec, _ = plan.solve(ks)
+    w_o, _, _ = oracle.solve(mat, G, uepm.NONLOCAL | uepm.SOC, 16, ks, nthreads=8)
+    err = np.abs(ev - w_o).max()
+    print(f"dim {2 * ng}: max |dE| = {err * 1e3:.3e} meV")
+    assert err <= MIXED_TOL
+    H = oracle.assemble(mat, G, ks[1], nl=True, soc=True)
+    X = vec[1].T
+    assert np.abs(X.conj().T @ X - np.eye(16)).max() <= 1e-10
+    assert np.linalg.norm(H @ X - X * ev[1], axis=0).max() <= 1e-3
     plan.close()
 
 

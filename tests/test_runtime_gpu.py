@@ -33,10 +33,13 @@ def test_config_validation_and_workspace(gpu_ctx):
     p.set_config(tridiag=1, tri_block=256, bt_bands=8, max_chunk=512, lanes=1)
     assert p.workspace_bytes(5000) == p.workspace_bytes(512)
     q = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 274
+    q.set_config(tridiag=2)  # the register-resident reduction takes dims up to 288
+    r = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(11), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 338
     with pytest.raises(UepmError):
-        q.set_config(tridiag=2)  # register-resident reduction is for dim <= 256
+        r.set_config(tridiag=2)
     p.close()
     q.close()
+    r.close()
 
 
 @pytest.mark.parametrize("flags", [api.NONLOCAL | api.VECTORS, api.NONLOCAL | api.VECTORS | api.MIXED])
@@ -94,13 +97,13 @@ def test_tuned_plan_keeps_results(gpu_ctx):
 
 
 def test_tuned_plan_above_the_register_grids(gpu_ctx):
-    """Above dim 256 the tuner chooses between the LDS-blocked reduction alone
+    """Above dim 288 the tuner chooses between the LDS-blocked reduction alone
     and its leading steps followed by the register grids; every candidate
     gives the same eigenvalues to the precision form's bar."""
-    print("case: autotune Ge SOC n=10 mixed (dim 274), 256 k", flush=True)
+    print("case: autotune Ge SOC n=11 mixed (dim 338), 256 k", flush=True)
     ks = api.bz_sample(11, 256)
-    p = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(10), api.NONLOCAL | api.SOC | api.MIXED, 16)
-    assert p.dim == 274
+    p = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(11), api.NONLOCAL | api.SOC | api.MIXED, 16)
+    assert p.dim == 338
     ev0, _, _ = p.solve(ks)
     dk = _dev(gpu_ctx, ks)
     chosen = p.tune(dk.ptr, len(ks))

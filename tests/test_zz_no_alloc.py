@@ -39,7 +39,8 @@ def _solve(fn, plan, ks, nb, dim, vectors, gradients, stream):
 @pytest.mark.parametrize("sym,n,flags,nb", [
     ("Si", 12, api.NONLOCAL | api.MIXED | api.VECTORS, 12),  # N = 181, precision form, vectors
     ("Si", 12, api.NONLOCAL | api.VECTORS | api.GRADIENTS, 12),  # FP64 with gradients
-    ("Ge", 10, api.NONLOCAL | api.SOC | api.MIXED, 16),  # dim 274: blocked leading steps
+    ("Ge", 10, api.NONLOCAL | api.SOC | api.MIXED, 16),  # dim 274: register grids from the 288-wide one
+    ("Ge", 11, api.NONLOCAL | api.SOC | api.MIXED, 16),  # dim 338: blocked leading steps, then the grids
 ])
 def test_noalloc_equals_solve_device(gpu_ctx, sym, n, flags, nb):
     print(f"case: no-allocation solve vs uepm_solve_device, {sym} n={n} flags {flags} nb {nb}", flush=True)

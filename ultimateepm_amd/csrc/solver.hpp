@@ -68,9 +68,9 @@ int tridiag_reg_split(int n);
 size_t tridiag_reg_work_bytes(int n);
 hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, float2* Vout, double* dd, double* de,
                                   float2* dtau, float2* work, hipStream_t s);
-// Complex64 register grids after the blocked reduction for n > 256: the
+// Complex64 register grids after the blocked reduction for n > 288: the
 // trailing size they take over (a panel boundary of the blocked kernel; n
-// itself for n <= 256) and the launch on a complex64
+// itself for n <= 288) and the launch on a complex64
 // block left in place by the blocked kernel (arguments as for the FP64 form
 // below; the hand-over workspace is tridiag_reg_work_bytes(m)).
 int tridiag_reg_c64_tail(int n);
@@ -81,14 +81,15 @@ hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int 
 // size it takes over for a matrix of size n (n itself when n <= 192), its
 // hand-over workspace per k-point (bytes), and the
 // launch on a block of size m whose element (0, 0) sits at H (leading
-// dimension ld, per-k stride hk); reflectors, d, e, tau at the block's offset
-// of the full matrix's outputs (V may alias H: each stage loads its block
-// before it writes).
+// dimension ld, per-k stride hk; hermitian_full: both triangles hold the
+// Hermitian block, else only its lower triangle); reflectors, d, e, tau at the
+// block's offset of the full matrix's outputs (V may alias H: each stage loads
+// its block before it writes, and writes only below the sub-diagonal).
 int tridiag_reg_c128_tail(int n);
 size_t tridiag_reg_c128_work_bytes(int m);
-hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv,
-                                   int64_t vk, double* dd, double* de, double2* dtau, int64_t dk, double2* work,
-                                   hipStream_t s);
+hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, bool hermitian_full, int64_t nk,
+                                   double2* V, int ldv, int64_t vk, double* dd, double* de, double2* dtau, int64_t dk,
+                                   double2* work, hipStream_t s);
 
 // Eigenvectors of H from those of T: cluster re-orthogonalisation of Z
 // ([k][band][row], n x nb per k), then x = Q z with the reflectors of the

@@ -1,6 +1,6 @@
 // tridiag_reg.hip — register-resident Householder reduction of a Hermitian
 // H(k) to real symmetric tridiagonal form, one 512-lane workgroup per
-// k-point (gfx950): complex64 arithmetic for the precision form (dim <= 256),
+// k-point (gfx950): complex64 arithmetic for the precision form (dim <= 288),
 // complex128 for the FP64 form (dim <= 192), and a chain of shrinking grids
 // (below).
 //
@@ -12,7 +12,9 @@
 // (l, m) that can hold i >= j are stored: 42 complex values per lane at
 // dim <= 192.  The diagonal is stored halved and the strict upper part of a
 // block as zero, so the symmetric product treats every stored element alike:
-// A(i, j) feeds row i (A v_j) and column j (conj(A) v_i).
+// A(i, j) feeds row i (A v_j) and column j (conj(A) v_i).  A grid whose blocks
+// do not all fit its register budget keeps its last NL blocks in LDS instead,
+// in lane-private slots (slot-major, so a wave's accesses are consecutive).
 //
 // Step s (LAPACK xHETD2, lower, restated; H_s = I - tau v v^H, v[s+1] = 1):
 //   a. the lanes of column s put x = A(s+1:, s) in LDS              | barrier
@@ -125,18 +127,15 @@ __device__ __forceinline__ VT store_c(R x, R y) {
 }
 
 
-// Stored blocks of the cyclic lower triangle.
+// Stored blocks of the cyclic lower triangle.  With PR = 2 PC and LC = 2 LR
+// (the only shape used) row block l stores block columns 0..2l+1, so the
+// offsets have closed forms: everything below folds to constants once the
+// loops over l and m are unrolled, which keeps the slot array in registers.
 template <int PR, int PC, int LR, int LC>
 struct RegLayout {
-    static constexpr int mmax(int l) {
-        const int v = (PR * l + PR - 1) / PC;
-        return v < LC - 1 ? v : LC - 1;
-    }
-    static constexpr int base(int l) {
-        int s = 0;
-        for (int q = 0; q < l; ++q) s += mmax(q) + 1;
-        return s;
-    }
+    static_assert(PR == 2 * PC && LC == 2 * LR, "closed forms below assume a 2:1 grid with square coverage");
+    static constexpr int mmax(int l) { return 2 * l + 1; }  // = min((PR l + PR - 1) / PC, LC - 1)
+    static constexpr int base(int l) { return l * (l + 1); }  // sum of mmax(q) + 1 over q < l
     static constexpr int count() { return base(LR); }
     static constexpr int idx(int l, int m) { return base(l) + m; }
     // first block column of row block l that may hold i <= j for some lane
@@ -145,23 +144,23 @@ struct RegLayout {
         return t <= 0 ? 0 : (t + PC - 1) / PC;
     }
     static constexpr int XC = PR / PC + 1;  // crossing blocks per row block (upper bound)
-    // first row block l with block (l, m) stored
-    static constexpr int lmin(int m) {
-        int l = 0;
-        while (l < LR && mmax(l) < m) ++l;
-        return l;
-    }
+    // first row block l with block (l, m) stored: the smallest l with 2l + 1 >= m
+    static constexpr int lmin(int m) { return m / 2; }
 };
 
 }  // namespace
 
 // Stage arguments: steps s < s_end run here (s_end >= n - 1: the whole
 // reduction); when s_end < n - 1 the updated trailing block A[s_end:, s_end:]
-// (lower triangle, full diagonal) goes to Bout (complex64, leading dimension
-// ldb, ldb * ldb per k) for a smaller kernel to continue.  Input (leading
+// (lower triangle, full diagonal) goes to Bout row by row (element (i, j) at
+// j + i ldb, ldb * ldb per k) for a smaller grid to continue.  IN says how
+// the lower triangle of the input is read: 0, column-major (element (i, j) at
+// i + j ld); 1, as the conjugate of the upper triangle of a full Hermitian
+// matrix (element (j, i) at j + i ld); 2, row by row as a hand-over block.  Input (leading
 // dimension ld), reflectors (ldv) and d, e, tau of k-point kk start at
 // kk * h_kstride, kk * v_kstride and kk * de_kstride.
-template <int PR, int PC, int LR, int LC, int MINWG, typename R, typename VT, typename THin, bool LOW = false>
+template <int PR, int PC, int LR, int LC, int MINWG, typename R, typename VT, typename THin, bool LOW = false,
+          int NL = 0, int IN = 0>
 __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* __restrict__ Hin, int64_t h_kstride,
                                                                     int n, int ld, VT* __restrict__ Vout,
                                                                     int ldv, int64_t v_kstride,
@@ -189,6 +188,11 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     __shared__ f2 CP[NW][NX];  // column sums, one row per wave
     __shared__ f2 Y[NX];       // y = A v, zero at rows <= s and >= n
     __shared__ R Q[NW];    // per-wave partial v^H A v
+    // the last NL stored blocks of every lane live in LDS (lane-private slots,
+    // slot-major so that a wave's accesses are consecutive), the others in VGPRs
+    static_assert(NL >= 0 && NL < NS, "LDS slots");
+    constexpr int NR = NS - NL;
+    __shared__ f2 AL[NL > 0 ? NL : 1][NL > 0 ? BS : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int c = tid % PC, r = tid / PC;
@@ -201,7 +205,21 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     const int s_stop = s_end < n - 1 ? s_end : n - 1;
 
     // ---- load the lower triangle (diagonal halved, strict upper zero) ----
-    f2 A[NS];
+    f2 A[NR];
+    // slot i of this lane (i is a compile-time constant wherever these are used)
+    auto aget = [&](int i) -> f2 {
+        if constexpr (NL > 0)
+            if (i >= NR) return AL[i - NR][tid];
+        return A[i];
+    };
+    auto aset = [&](int i, f2 v) {
+        if constexpr (NL > 0)
+            if (i >= NR) {
+                AL[i - NR][tid] = v;
+                return;
+            }
+        A[i] = v;
+    };
 #pragma unroll
     for (int l = 0; l < LR; ++l) {
 #pragma unroll
@@ -209,11 +227,13 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
             const int i = r + PR * l, j = c + PC * m;
             f2 a = mk<R>(0, 0);
             if (i < n && j < n && i >= j) {
-                const THin h = H[i + (int64_t)j * ld];
+                // a wave's 16 columns j are consecutive: IN 1 and 2 read them as
+                // contiguous runs (row i of the stored triangle)
+                const THin h = H[IN == 0 ? i + (int64_t)j * ld : j + (int64_t)i * ld];
                 const R f = (i == j) ? R(0.5) : R(1);
-                a = mk<R>((R)h.x * f, (R)h.y * f);
+                a = mk<R>((R)h.x * f, (IN == 1 ? -(R)h.y : (R)h.y) * f);
             }
-            A[Lay::idx(l, m)] = a;
+            aset(Lay::idx(l, m), a);
         }
     }
     // update weights of the blocks that straddle the diagonal: 1 below, 1/2 on, 0 above
@@ -242,7 +262,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                     for (int l = 0; l < LR; ++l) {
                         const int i = r + PR * l;
                         if (m <= Lay::mmax(l)) {
-                            const f2 a = A[Lay::idx(l, m)];
+                            const f2 a = aget(Lay::idx(l, m));
                             X[i] = sel(i > s, a, zero);
                             XD[i] = a;
                         } else {
@@ -325,7 +345,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                 const f2 vcm = LOW ? vcol(m) : vc[LOW ? 0 : m];
 #pragma unroll
                 for (int l = Lay::lmin(m); l < LR; ++l) {
-                    const f2 a = A[Lay::idx(l, m)];
+                    const f2 a = aget(Lay::idx(l, m));
                     fma_ab(rp[l], a, vcm);
                     fma_cab(cp[mc], a, vr[l]);
                 }
@@ -420,7 +440,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                         vi = mk(h * vi.x, h * vi.y);
                         wi = mk(h * wi.x, h * wi.y);
                     }
-                    f2& a = A[Lay::idx(l, m)];
+                    f2 a = aget(Lay::idx(l, m));
                     a.x = fma(-vi.x, wcm.x, a.x);
                     a.x = fma(-vi.y, wcm.y, a.x);
                     a.x = fma(-wi.x, vcm.x, a.x);
@@ -429,6 +449,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                     a.y = fma(vi.x, wcm.y, a.y);
                     a.y = fma(-wi.y, vcm.x, a.y);
                     a.y = fma(wi.x, vcm.y, a.y);
+                    aset(Lay::idx(l, m), a);
                 }
             }
         }
@@ -442,9 +463,9 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
             for (int m = 0; m <= Lay::mmax(l); ++m) {
                 const int i = r + PR * l, j = c + PC * m;
                 if (i < n && j >= s_stop && i >= j) {
-                    const f2 a = A[Lay::idx(l, m)];
+                    const f2 a = aget(Lay::idx(l, m));
                     const R f = (i == j) ? R(2) : R(1);
-                    B[(i - s_stop) + (int64_t)(j - s_stop) * ldb] = store_c<VT>(a.x * f, a.y * f);
+                    B[(j - s_stop) + (int64_t)(i - s_stop) * ldb] = store_c<VT>(a.x * f, a.y * f);
                 }
             }
         return;
@@ -458,7 +479,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
             if (m == ms)
 #pragma unroll
                 for (int l = 0; l < LR; ++l)
-                    if (m <= Lay::mmax(l) && c == cs && r + PR * l == s) d[s] = 2.0 * (double)A[Lay::idx(l, m)].x;
+                    if (m <= Lay::mmax(l) && c == cs && r + PR * l == s) d[s] = 2.0 * (double)aget(Lay::idx(l, m)).x;
         if (tid == 0) {
             e[s] = 0.0;
             tau_o[s] = store_c<VT>(R(0), R(0));
@@ -466,20 +487,165 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     }
 }
 
-// n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12; n <= 256: 8 x 16
-// (rows x columns of the 32 x 16 grid)
-int tridiag_reg_max_n() { return 256; }
+// One wave per k-point for a block of size n <= 64 (complex64; the chain's
+// last stage, or the whole reduction of a small matrix).  Lane i keeps row i
+// of the Hermitian block in VGPRs, both triangles, in shifted columns: at step
+// s, R[t] = A(i, s + t).  Column s is then R[0] on every lane, the product
+// y = A v is a local dot product of the row with v (read from LDS, where each
+// lane publishes its v_i and w_i), and the rank-2 update writes R[t - 1] from
+// R[t], which moves the columns along at no cost.  No barriers: the wave is
+// the workgroup.  Columns are processed in blocks of 8, skipped once they are
+// past the live part (those registers are never read again).  Same outputs
+// as the grid kernels (d, e, tau, reflectors below the sub-diagonal of column
+// s of Vout).  The block is read as the grids read it (IN: 0 lower triangle
+// by columns, 1 conjugate of the upper triangle of a full Hermitian matrix,
+// 2 lower triangle by rows).
+template <int IN, typename THin>
+__global__ __launch_bounds__(64, 2) void tridiag_wave64_kernel(const THin* __restrict__ Hin, int64_t h_kstride, int n,
+                                                               int ld, float2* __restrict__ Vout, int ldv,
+                                                               int64_t v_kstride, double* __restrict__ dout,
+                                                               double* __restrict__ eout, float2* __restrict__ tauout,
+                                                               int64_t de_kstride) {
+    using f2 = c2<float>;
+    constexpr int W = 64;
+    // (v, w) of lane j at VW[j]; entries W..2W-1 stay zero (columns past the block)
+    __shared__ float4 VW[2 * W];
+    const int i = threadIdx.x;
+    const int64_t kk = blockIdx.x;
+    const THin* H = Hin + kk * h_kstride;
+    float2* Vk = Vout + kk * v_kstride;
+    double* d = dout + kk * de_kstride;
+    double* e = eout + kk * de_kstride;
+    float2* tau_o = tauout + kk * de_kstride;
+    const f2 zero = mk<float>(0, 0), one = mk<float>(1, 0);
+    VW[W + i] = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    // ---- row i of the block (zero outside it) ----
+    f2 R[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        f2 a = zero;
+        if (i < n && j < n) {
+            // lower element A(r, c), r >= c, as the grids read it
+            const int r = j <= i ? i : j, c = j <= i ? j : i;
+            const THin h = H[IN == 0 ? r + (int64_t)c * ld : c + (int64_t)r * ld];
+            const float hx = (float)h.x, hy = IN == 1 ? -(float)h.y : (float)h.y;
+            a = j <= i ? mk(hx, hy) : mk(hx, -hy);  // A(i, j) = conj(A(j, i)) above the diagonal
+        }
+        R[j] = a;
+    }
+
+    for (int s = 0; s < n - 1; ++s) {
+        const int L = n - 1 - s;  // live columns s + 1 .. n - 1 sit in R[1 .. L]
+        // ---- reflector from x = A(s+1:, s), redundantly in every lane ----
+        const f2 x = R[0];
+        const bool below = i > s + 1 && i < n;
+        const float xn2 = wave::sum(below ? fmaf(x.x, x.x, x.y * x.y) : 0.f);
+        const float alx = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x.x), s + 1));
+        const float aly = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x.y), s + 1));
+        const float dss = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x.x), s));
+        float beta = alx, tr = 0, ti = 0, sr = 1, si = 0;
+        if (!(xn2 == 0.f && aly == 0.f)) {
+            const float nrm = sqrtf(fmaf(alx, alx, fmaf(aly, aly, xn2)));
+            beta = alx >= 0.f ? -nrm : nrm;
+            const float ib = rcp_full(beta);
+            tr = (beta - alx) * ib;
+            ti = -aly * ib;
+            const float br = alx - beta, iden = rcp_full(fmaf(br, br, aly * aly));
+            sr = br * iden;
+            si = -aly * iden;
+        }
+        const f2 tau = mk(tr, ti), sc = mk(sr, si);
+        if (i == 0) {
+            d[s] = (double)dss;
+            e[s] = (double)beta;
+            tau_o[s] = make_float2(tau.x, tau.y);
+        }
+        const f2 v = i == s + 1 ? one : (below ? mul(x, sc) : zero);
+        if (below) Vk[i + (int64_t)s * ldv] = make_float2(v.x, v.y);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        VW[i] = make_float4(v.x, v.y, 0.f, 0.f);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // ---- y = A v over the live columns (v_j = 0 for j <= s) ----
+        f2 y = zero;
+#pragma unroll
+        for (int b = 0; b < W / 8; ++b) {
+            if (8 * b + 1 <= L) {  // wave-uniform
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int t = 8 * b + u + 1;
+                    if (t < W) {
+                        const float4 q = VW[s + t];
+                        fma_ab(y, R[t], mk(q.x, q.y));
+                    }
+                }
+            }
+        }
+        // ---- w = tau y + alpha v, alpha = -(1/2) |tau|^2 v^H A v ----
+        const float vav = wave::sum(fmaf(v.x, y.x, v.y * y.y));
+        const float alpha = -0.5f * (tau.x * tau.x + tau.y * tau.y) * vav;
+        f2 w = mul(tau, y);
+        w.x = fmaf(alpha, v.x, w.x);
+        w.y = fmaf(alpha, v.y, w.y);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        VW[i] = make_float4(v.x, v.y, w.x, w.y);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // ---- A -= v w^H + w v^H on the live columns, shifted down by one ----
+#pragma unroll
+        for (int b = 0; b < W / 8; ++b) {
+            if (8 * b + 1 <= L) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int t = 8 * b + u + 1;
+                    if (t < W) {
+                        const float4 q = VW[s + t];  // (v_j, w_j), j = s + t
+                        f2 a = R[t];
+                        a.x = fmaf(-v.x, q.z, a.x);
+                        a.x = fmaf(-v.y, q.w, a.x);
+                        a.x = fmaf(-w.x, q.x, a.x);
+                        a.x = fmaf(-w.y, q.y, a.x);
+                        a.y = fmaf(-v.y, q.z, a.y);
+                        a.y = fmaf(v.x, q.w, a.y);
+                        a.y = fmaf(-w.y, q.x, a.y);
+                        a.y = fmaf(w.x, q.y, a.y);
+                        R[t - 1] = a;
+                    }
+                }
+            }
+        }
+    }
+    // last diagonal element
+    {
+        const int s = n - 1;
+        const float dss = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, R[0].x), s));
+        if (i == 0) {
+            d[s] = (double)dss;
+            e[s] = 0.0;
+            tau_o[s] = make_float2(0.f, 0.f);
+        }
+    }
+}
+
+// n <= 64: 2 x 4 blocks; n <= 128: 4 x 8; n <= 192: 6 x 12; n <= 256: 8 x 16;
+// n <= 288: 9 x 18 (rows x columns of the 32 x 16 grid)
+int tridiag_reg_max_n() { return 288; }
 
 // Hand-over between grids.  The late steps of a large grid carry its full
 // per-step overhead for a small trailing matrix, so each grid stops where the
-// next smaller one can take the trailing block: 256-grid (8 x 16 blocks) ->
-// 224 (7 x 14) -> 192 (6 x 12) -> 160 (5 x 10) -> 128 (4 x 8) -> 96 (3 x 6) ->
-// 64 (2 x 4).  Register budgets (workgroups per CU): 256 and 224
-// low-register, 256 and 223 VGPRs (1); 192 and 160 low-register, 128 VGPRs
-// (2); 128 at 128 VGPRs (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
+// next smaller one can take the trailing block: 288-grid (9 x 18 blocks) ->
+// 256 (8 x 16) -> 224 (7 x 14) -> 192 (6 x 12) -> 160 (5 x 10) -> 128 (4 x 8)
+// -> 96 (3 x 6) -> 64 (2 x 4).  Register budgets (workgroups per CU): 288
+// low-register, 256 VGPRs with 16 of its 90 blocks in LDS (1); 256 and 224
+// low-register, 256 and 223 VGPRs (1); 192 low-register, 128 VGPRs with 14 of
+// its 42 blocks in LDS (2); 160 low-register, 128 VGPRs (2); 128 at 128 VGPRs
+// (2); 96 low-register, 80 (3); 64 low-register, 63 (4).
 namespace {
-constexpr int kNumGrids = 7;
-constexpr int kGrid[kNumGrids] = {256, 224, 192, 160, 128, 96, 64};
+constexpr int kNumGrids = 8;
+constexpr int kGrid[kNumGrids] = {288, 256, 224, 192, 160, 128, 96, 64};
 // index of the smallest grid that holds n
 int grid_of(int n) {
     int g = 0;
@@ -502,38 +668,49 @@ size_t tridiag_reg_work_bytes(int n) {
 }
 
 namespace {
-template <typename R, typename VT, int LR, int LC, int MINWG, typename THin, bool LOW = false>
+template <int IN, typename THin>
+void wave64_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, float2* V, int ldv, int64_t v_kstride,
+                  double* dd, double* de, float2* dtau, int64_t de_kstride, hipStream_t s) {
+    hipLaunchKernelGGL((tridiag_wave64_kernel<IN, THin>), dim3((unsigned)nk), dim3(64), 0, s, H, h_kstride, n, ld, V, ldv,
+                       v_kstride, dd, de, dtau, de_kstride);
+}
+
+template <typename R, typename VT, int LR, int LC, int MINWG, int IN, typename THin, bool LOW = false, int NL = 0>
 void reg_stage(const THin* H, int64_t h_kstride, int n, int ld, int64_t nk, VT* V, int ldv, int64_t v_kstride,
                double* dd, double* de, VT* dtau, int64_t de_kstride, int s_end, VT* B, int ldb, hipStream_t s) {
-    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, R, VT, THin, LOW>), dim3((unsigned)nk), dim3(512), 0,
+    hipLaunchKernelGGL((tridiag_reg_kernel<32, 16, LR, LC, MINWG, R, VT, THin, LOW, NL, IN>), dim3((unsigned)nk), dim3(512), 0,
                        s, H, h_kstride, n, ld, V, ldv, v_kstride, dd, de, dtau, de_kstride, s_end, B, ldb);
 }
 
-template <typename THin>
+constexpr int NL288 = 16, NL192 = 14;  // blocks held in LDS by the 288- and 192-wide grids
+
+template <typename THin, int IN>
 void grid_stage(int g, const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk,
                 double* dd, double* de, float2* dtau, int64_t dk, int s_end, float2* B, int ldb, hipStream_t s) {
     switch (g) {
-        case 0: reg_stage<float, float2, 8, 16, 2, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 1: reg_stage<float, float2, 7, 14, 2, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 2: reg_stage<float, float2, 6, 12, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<float, float2, 5, 10, 4, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 4: reg_stage<float, float2, 4, 8, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 5: reg_stage<float, float2, 3, 6, 6, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        default: reg_stage<float, float2, 2, 4, 8, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 0: reg_stage<float, float2, 9, 18, 2, IN, THin, true, NL288>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<float, float2, 8, 16, 2, IN, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 2: reg_stage<float, float2, 7, 14, 2, IN, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<float, float2, 6, 12, 4, IN, THin, true, NL192>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 4: reg_stage<float, float2, 5, 10, 4, IN, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 5: reg_stage<float, float2, 4, 8, 4, IN, THin>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 6: reg_stage<float, float2, 3, 6, 6, IN, THin, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: wave64_stage<IN, THin>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s); break;  // last: s_end >= m - 1
     }
 }
 }  // namespace
 
 namespace {
 // The complex64 chain on a block of size m whose element (0, 0) sits at H
-// (leading dimension ld, per-k stride hk); outputs at the block's offset of
-// the full matrix's V (ldv, stride vk) and d, e, tau (stride dk).
-template <typename THin>
+// (leading dimension ld, per-k stride hk; read as IN says, IN = 0 or 1);
+// outputs at the block's offset of the full matrix's V (ldv, stride vk) and
+// d, e, tau (stride dk).
+template <int IN, typename THin>
 hipError_t c64_chain(const THin* H, int64_t hk, int m, int ld, int64_t nk, float2* V, int ldv, int64_t vk, double* dd,
                      double* de, float2* dtau, int64_t dk, float2* work, hipStream_t s) {
     const int t1 = tridiag_reg_split(m);
     if (!t1) {
-        grid_stage(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m, (float2*)nullptr, 0, s);
+        grid_stage<THin, IN>(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m, (float2*)nullptr, 0, s);
         return hipGetLastError();
     }
     if (!work) return hipErrorInvalidValue;
@@ -541,12 +718,12 @@ hipError_t c64_chain(const THin* H, int64_t hk, int m, int ld, int64_t nk, float
     // its size, in ping-pong blocks (the block of size t lives at
     // buf[i] + k * t * t; a stage reads all of its block before it writes the next)
     float2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
-    grid_stage(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m - t1, buf[0], t1, s);
+    grid_stage<THin, IN>(grid_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m - t1, buf[0], t1, s);
     int r = t1, cur = 0;
     while (true) {
         const int off = m - r;
         const int t = tridiag_reg_split(r);
-        grid_stage(grid_of(r), (const float2*)buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv,
+        grid_stage<float2, 2>(grid_of(r), (const float2*)buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv,
                    vk, dd + off, de + off, dtau + off, dk, t ? r - t : r, t ? buf[cur ^ 1] : (float2*)nullptr, t,
                    s);
         if (!t) break;
@@ -561,7 +738,7 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
                                   float2* dtau, float2* work, hipStream_t s) {
     if (n < 1 || n > kGrid[0] || nk <= 0) return hipErrorInvalidValue;
     const int64_t hk = (int64_t)ld * n;
-    return c64_chain(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, work, s);
+    return c64_chain<1>(dH, hk, n, ld, nk, Vout, ld, hk, dd, de, dtau, n, work, s);
 }
 
 int tridiag_reg_c64_tail(int n) {
@@ -576,7 +753,7 @@ hipError_t launch_tridiag_reg_c64_block(const float2* H, int64_t hk, int m, int 
                                         int64_t vk, double* dd, double* de, float2* dtau, int64_t dk, float2* work,
                                         hipStream_t s) {
     if (m < 1 || m > kGrid[0] || nk <= 0) return hipErrorInvalidValue;
-    return c64_chain(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, work, s);
+    return c64_chain<0>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, work, s);
 }
 
 // ---- FP64: grids 192 (6 x 12, low-register, 256 VGPRs with a small spill), 160
@@ -595,14 +772,16 @@ int split64(int n) {
     const int g = grid64_of(n);
     return g < kNumGrids64 - 1 ? kGrid64[g + 1] : 0;
 }
+constexpr int NL64_192 = 10, NL64_96 = 2;  // blocks held in LDS by the 192- and 96-wide FP64 grids
+template <int IN>
 void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv, int64_t vk,
                   double* dd, double* de, double2* dtau, int64_t dk, int s_end, double2* B, int ldb, hipStream_t s) {
     switch (g) {
-        case 0: reg_stage<double, double2, 6, 12, 1, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 1: reg_stage<double, double2, 5, 10, 1, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 2: reg_stage<double, double2, 4, 8, 1>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        case 3: reg_stage<double, double2, 3, 6, 4, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
-        default: reg_stage<double, double2, 2, 4, 4>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 0: reg_stage<double, double2, 6, 12, 1, IN, double2, true, NL64_192>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 1: reg_stage<double, double2, 5, 10, 1, IN, double2, true>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 2: reg_stage<double, double2, 4, 8, 1, IN, double2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        case 3: reg_stage<double, double2, 3, 6, 4, IN, double2, true, NL64_96>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
+        default: reg_stage<double, double2, 2, 4, 4, IN, double2>(H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s); break;
     }
 }
 }  // namespace
@@ -623,23 +802,31 @@ size_t tridiag_reg_c128_work_bytes(int m) {
     return sizeof(double2) * ((size_t)t1 * t1 + (size_t)t2 * t2);
 }
 
-hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, int64_t nk, double2* V, int ldv,
-                                   int64_t vk, double* dd, double* de, double2* dtau, int64_t dk, double2* work,
-                                   hipStream_t s) {
+hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, bool hermitian_full, int64_t nk,
+                                   double2* V, int ldv, int64_t vk, double* dd, double* de, double2* dtau, int64_t dk,
+                                   double2* work, hipStream_t s) {
     if (m < 1 || m > kGrid64[0] || nk <= 0) return hipErrorInvalidValue;
+    // the first grid reads the block's rows (the conjugate upper triangle) when
+    // the whole Hermitian matrix is there, its lower columns otherwise
+    auto first = [&](int s_end, double2* B, int ldb) {
+        if (hermitian_full)
+            grid64_stage<1>(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s);
+        else
+            grid64_stage<0>(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, s_end, B, ldb, s);
+    };
     const int t1 = split64(m);
     if (!t1) {
-        grid64_stage(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m, (double2*)nullptr, 0, s);
+        first(m, nullptr, 0);
         return hipGetLastError();
     }
     if (!work) return hipErrorInvalidValue;
     double2* buf[2] = {work, work + nk * (int64_t)t1 * t1};
-    grid64_stage(grid64_of(m), H, hk, m, ld, nk, V, ldv, vk, dd, de, dtau, dk, m - t1, buf[0], t1, s);
+    first(m - t1, buf[0], t1);
     int r = t1, cur = 0;
     while (true) {
         const int off = m - r;
         const int t = split64(r);
-        grid64_stage(grid64_of(r), buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv, vk, dd + off,
+        grid64_stage<2>(grid64_of(r), buf[cur], (int64_t)r * r, r, r, nk, V + off + (int64_t)off * ldv, ldv, vk, dd + off,
                      de + off, dtau + off, dk, t ? r - t : r, t ? buf[cur ^ 1] : (double2*)nullptr, t, s);
         if (!t) break;
         r = t;

@@ -53,7 +53,7 @@ int64_t plan_chunk(const uepm_plan* p, int64_t nk) {
 }
 
 namespace {
-// Trailing size of the precision form's blocked reduction (n > 256) that the
+// Trailing size of the precision form's blocked reduction (n > 288) that the
 // register grids take over; 0 when the configuration asks for the blocked kernel.
 int mixed_c64_tail(const uepm_plan* p) {
     if (p->tri_reg || p->cfg.tridiag == 1 || p->dim <= uepm::tridiag_reg_max_n()) return 0;
@@ -273,7 +273,7 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
                     HIP_OK(uepm::launch_tridiag_reg_c64(Hc, n, n, c, Vc, w.d, w.e,
                                                         w.tauf, w.S32, cs));
                 else {
-                    // n > 256: the blocked reduction for the leading steps, the register grids
+                    // n > 288: the blocked reduction for the leading steps, the register grids
                     // for the trailing block it leaves in place in Vc
                     const int t = mixed_c64_tail(p);
                     const int s0 = n - t;
@@ -323,7 +323,7 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
                                                             t > 0 ? s0 : n - 1));
                 if (t > 0) {
                     double2* B = Hc + s0 + (int64_t)s0 * n;
-                    HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, c, B, n, nn, w.d + s0,
+                    HIP_OK(uepm::launch_tridiag_reg_c128(B, nn, t, n, t == n, c, B, n, nn, w.d + s0,
                                                          w.e + s0, w.tau + s0, n,
                                                          w.S, cs));
                 }
