@@ -753,6 +753,8 @@ def main() -> int:
         d = res["mixed"]["evals"] - res["fp64"]["evals"]
         extra["mixed_vs_fp64_max_meV"] = float(np.abs(d).max() * 1e3)
         extra["mixed_vs_fp64_rms_meV"] = float(np.sqrt(np.mean(d ** 2)) * 1e3)
+        # the precision form's bar: every eigenvalue within 0.1 meV of the FP64 form
+        extra["mixed_within_0p1_meV"] = bool(np.abs(d).max() <= 1e-4)
         extra["mixed_over_fp64"] = round(extra["mixed_kpts_per_s"] / extra["fp64_kpts_per_s"], 3)
 
     out = {

@@ -89,6 +89,27 @@ def test_large_dimensions_vs_oracle(gpu_ctx, oracle, sym, n, soc):
     p.close()
 
 
+def test_precision_form_bar_over_a_large_batch(gpu_ctx):
+    """Rare k-points decide the precision form's worst case: over 65,536 seeded
+    k-points of the headline workload (Si non-local N = 181, 12 bands) every
+    eigenvalue of the precision form is within 0.1 meV of the FP64 form's (the
+    FP64 form is pinned to the oracle by the other tests)."""
+    print("case: Si n=12 N=181, 65536 k, precision vs FP64 form", flush=True)
+    ks = api.bz_sample(20261203, 65536)
+    dk = _dev(gpu_ctx, ks)
+    out = {}
+    for name, flags in (("fp64", api.NONLOCAL), ("mixed", api.NONLOCAL | api.MIXED)):
+        p = api.Plan(gpu_ctx, load("Si"), api.basis_vectors(12), flags, 12)
+        de = api.DeviceArray(gpu_ctx, (len(ks), 12))
+        p.solve_device(dk.ptr, len(ks), de.ptr)
+        out[name] = de.numpy()
+        assert p.status() == (0, -1)
+        p.close()
+    d = np.abs(out["mixed"] - out["fp64"])
+    print(f"max {d.max() * 1e3:.3e} meV, rms {np.sqrt(np.mean(d ** 2)) * 1e3:.3e} meV", flush=True)
+    assert d.max() <= MIXED_TOL
+
+
 def test_batching_workload_graph_vs_oracle(gpu_ctx, oracle):
     """All 28 plans of the batching workload in one graph (precision form, tuned per dimension)."""
     combos = [(m, n, soc) for m in ("Si", "Ge", "GaAs", "InP") for n in (6, 8, 10, 12) for soc in (False, True)

@@ -262,6 +262,202 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Blocked back-transform on MFMA (precision form).  The reflectors are applied
+// sixteen at a time as Q_b = H_j0 H_j0+1 ... H_j0+15 = I - V T V^H (compact WY),
+// the last block first: X <- X - V (T (V^H X)).  T is never formed: with
+// G = V^H V, T^-1 = U, the upper triangle with U_jj = 1 / tau_j and U_ij = G_ij
+// above the diagonal (the LAPACK xLARFT forward/columnwise recurrence
+// T_{0:j,j} = -tau_j T_{0:j,0:j} G_{0:j,j}, T_jj = tau_j, inverted), so
+// W' = T W is a back substitution with U.  One 256-lane workgroup per
+// k-point; X (n rows x 16 NT bands, complex64) lives in LDS.
+//   1. [G | W] = V^H [V | X] on v_mfma_f32_16x16x4f32 (complex products as
+//      four real ones), the rows split over the four waves; the per-wave
+//      partial sums are added in a fixed order, so results do not depend on
+//      scheduling;
+//   2. U W' = W by back substitution, one band per lane (a reflector with
+//      tau = 0 is the identity: its row of W' is zero);
+//   3. X -= V W' on MFMA, 16-row tiles over the waves.
+// Each reflector's column is read twice per block (steps 1 and 4) instead of
+// once per band group, and the arithmetic runs on the matrix cores.
+template <int NT>
+__global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __restrict__ V, int n, int ld,
+                                                         const float2* __restrict__ tauall,
+                                                         const double* __restrict__ Z, int nb,
+                                                         double2* __restrict__ X) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    constexpr int NBT = 16 * NT;
+    constexpr int XS = NBT + 1;  // padded row stride of X in LDS (complex elements)
+    extern __shared__ __align__(16) unsigned char wy_smem[];
+    float2* Xs = reinterpret_cast<float2*>(wy_smem);  // X[row][band] at row * XS + band
+    __shared__ f4 PG[4][2][64];      // per-wave partial G (re, im) in the MFMA output layout
+    __shared__ f4 PW[4][NT][2][64];  // per-wave partial W
+    __shared__ float2 Gs[16][17];    // G = V^H V
+    __shared__ float2 Ws[16][XS];    // W = V^H X
+    __shared__ float2 Wq[16][XS];    // W' = T W
+    __shared__ float2 tb[16];        // tau of the block's reflectors
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int64_t kk = blockIdx.x;
+    const float2* Vk = V + kk * (int64_t)ld * n;
+    const float2* tau = tauall + kk * n;
+    auto mfma = [](float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); };
+    // element `row` of v_j: 1 at row j + 1, the stored reflector below, zero above
+    // and for j past the last reflector (n - 2)
+    auto vload = [&](int row, int j) -> float2 {
+        if (j > n - 2 || row >= n || row <= j) return make_float2(0.f, 0.f);
+        if (row == j + 1) return make_float2(1.f, 0.f);
+        return Vk[row + (int64_t)j * ld];
+    };
+
+    // X = Z (real); bands past nb are zero
+    for (int e = tid; e < n * NBT; e += 256) {
+        const int b = e / n, row = e - b * n;
+        const float z = b < nb ? (float)Z[(kk * nb + b) * (int64_t)n + row] : 0.f;
+        Xs[row * XS + b] = make_float2(z, 0.f);
+    }
+    __syncthreads();
+
+    const int nblk = (n - 1 + 15) >> 4;  // reflectors 0 .. n - 2
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        const int j0 = 16 * blk, r0 = j0 + 1;
+        // ---- 1. partial [G | W] over this wave's 4-row chunks ----
+        const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+        f4 gre = z4, gim = z4, wre[NT], wim[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) wre[t] = wim[t] = z4;
+        const int nch = (n - r0 + 3) >> 2;
+        // four chunks per pass, their loads issued before the MFMAs that use them
+        for (int ch0 = wv; ch0 < nch; ch0 += 16) {
+            float2 vv[4], xx[4][NT];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int row = r0 + 4 * (ch0 + 4 * u) + l4;
+                // A[c][k] = conj(v_{j0+c}[row]) and B[k][c'] = v_{j0+c'}[row]: the same element per lane
+                vv[u] = vload(row, j0 + l16);
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    xx[u][t] = row < n ? Xs[row * XS + 16 * t + l16] : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const float2 v = vv[u];
+                gre = mfma(v.x, v.x, gre);
+                gre = mfma(v.y, v.y, gre);
+                gim = mfma(v.x, v.y, gim);
+                gim = mfma(-v.y, v.x, gim);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const float2 x = xx[u][t];
+                    wre[t] = mfma(v.x, x.x, wre[t]);
+                    wre[t] = mfma(v.y, x.y, wre[t]);
+                    wim[t] = mfma(v.x, x.y, wim[t]);
+                    wim[t] = mfma(-v.y, x.x, wim[t]);
+                }
+            }
+        }
+        PG[wv][0][lane] = gre;
+        PG[wv][1][lane] = gim;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            PW[wv][t][0][lane] = wre[t];
+            PW[wv][t][1][lane] = wim[t];
+        }
+        if (tid < 16) tb[tid] = j0 + tid <= n - 2 ? tau[j0 + tid] : make_float2(0.f, 0.f);
+        __syncthreads();
+        // sum the four waves' partials in a fixed order; output item q of lane L
+        // is element (4 (L / 16) + q, L % 16)
+        for (int fl = tid; fl < (2 + 2 * NT) * 64; fl += 256) {
+            const int f = fl >> 6, L = fl & 63;
+            f4 sum;
+            if (f < 2) {
+                sum = PG[0][f][L];
+                sum += PG[1][f][L];
+                sum += PG[2][f][L];
+                sum += PG[3][f][L];
+            } else {
+                const int t = (f - 2) >> 1, part = (f - 2) & 1;
+                sum = PW[0][t][part][L];
+                sum += PW[1][t][part][L];
+                sum += PW[2][t][part][L];
+                sum += PW[3][t][part][L];
+            }
+            const int c = L & 15;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 4 * (L >> 4) + q;
+                if (f < 2) {
+                    float* g = reinterpret_cast<float*>(&Gs[m][c]);
+                    g[f] = sum[q];
+                } else {
+                    const int t = (f - 2) >> 1, part = (f - 2) & 1;
+                    float* w = reinterpret_cast<float*>(&Ws[m][16 * t + c]);
+                    w[part] = sum[q];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- 2. W' = T W: back substitution U W' = W, one band per lane ----
+        for (int b = tid; b < NBT; b += 256) {
+            float2 wq[16];
+#pragma unroll
+            for (int i = 15; i >= 0; --i) {
+                float sr = Ws[i][b].x, si = Ws[i][b].y;
+#pragma unroll
+                for (int p = i + 1; p < 16; ++p) {
+                    const float2 u = Gs[i][p];
+                    sr = fmaf(-u.x, wq[p].x, fmaf(u.y, wq[p].y, sr));
+                    si = fmaf(-u.x, wq[p].y, fmaf(-u.y, wq[p].x, si));
+                }
+                const float2 t = tb[i];  // 1 / U_ii
+                wq[i] = make_float2(t.x * sr - t.y * si, t.x * si + t.y * sr);
+                Wq[i][b] = wq[i];
+            }
+        }
+        __syncthreads();
+        // ---- 3. X -= V W' on the rows below j0, 16-row tiles over the waves ----
+        const int ntile = (n - r0 + 15) >> 4;
+        for (int tt = wv; tt < ntile; tt += 4) {
+            const int R0 = r0 + 16 * tt;
+            float2 a[4];  // A[m][k] = v_{j0+k}[R0 + m]: lane (m = l16, k = 4 kc + l4)
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) a[kc] = vload(R0 + l16, j0 + 4 * kc + l4);
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                f4 dre, dim;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = R0 + 4 * l4 + q;
+                    const float2 x = row < n ? Xs[row * XS + 16 * t + l16] : make_float2(0.f, 0.f);
+                    dre[q] = x.x;
+                    dim[q] = x.y;
+                }
+#pragma unroll
+                for (int kc = 0; kc < 4; ++kc) {
+                    const float2 b = Wq[4 * kc + l4][16 * t + l16];
+                    dre = mfma(-a[kc].x, b.x, dre);
+                    dre = mfma(a[kc].y, b.y, dre);
+                    dim = mfma(-a[kc].x, b.y, dim);
+                    dim = mfma(-a[kc].y, b.x, dim);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int row = R0 + 4 * l4 + q;
+                    if (row < n) Xs[row * XS + 16 * t + l16] = make_float2(dre[q], dim[q]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // X0[k][band][row] in FP64
+    for (int e = tid; e < n * nb; e += 256) {
+        const int b = e / n, row = e - b * n;
+        const float2 x = Xs[row * XS + b];
+        X[(kk * nb + b) * (int64_t)n + row] = make_double2((double)x.x, (double)x.y);
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 namespace {
 // Bands per 16-lane row of apply_q_kernel (1, 2 or 4): as many as keep the
@@ -323,9 +519,33 @@ hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* d
                                          bands_per_group, s);
 }
 
+namespace {
+// dynamic LDS of apply_q_wy_kernel<NT> (X in LDS) and its static part
+size_t wy_lds_bytes(int n, int nt) { return sizeof(float2) * (size_t)n * (16 * nt + 1); }
+size_t wy_static_bytes(int nt) {
+    return 16 * 4 * 2 * 64 + 16 * 4 * nt * 2 * 64 + sizeof(float2) * (16 * 17 + 2 * 16 * (16 * nt + 1) + 16);
+}
+}  // namespace
+
+bool apply_q_wy_fits(int n, int nb, int lds_limit) {
+    if (nb > 32 || n < 2) return false;
+    const int nt = nb > 16 ? 2 : 1;
+    return wy_lds_bytes(n, nt) + wy_static_bytes(nt) <= (size_t)lds_limit;
+}
+
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
                               int bands_per_group, hipStream_t s) {
+    if (bands_per_group == 0) {
+        // the blocked MFMA back-transform
+        const int nt = nb > 16 ? 2 : 1;
+        const size_t shm = wy_lds_bytes(n, nt);
+        auto kern = nt == 1 ? apply_q_wy_kernel<1> : apply_q_wy_kernel<2>;
+        hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (err != hipSuccess) return err;
+        hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb, dX);
+        return hipGetLastError();
+    }
     return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, nullptr, nullptr, 0,
                                        bands_per_group, s);
 }

@@ -103,9 +103,12 @@ hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* d
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                               int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
                               int bands_per_group, hipStream_t s);
+// Precision form: bands_per_group 0 selects the blocked MFMA back-transform
+// (compact WY, sixteen reflectors per block), which needs apply_q_wy_fits.
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
                               int bands_per_group, hipStream_t s);
+bool apply_q_wy_fits(int n, int nb, int lds_limit);
 
 // Device-to-device copy (16-byte aligned pointers take the vector path).
 hipError_t launch_copy_bytes(void* dst, const void* src, uint64_t bytes, int num_cus, hipStream_t s);

@@ -10,9 +10,12 @@
 // i = r + 32 l, j = c + 16 m (cyclic in both directions, so every lane keeps
 // a share of the shrinking trailing matrix until the end).  Only blocks
 // (l, m) that can hold i >= j are stored: 42 complex values per lane at
-// dim <= 192.  The diagonal is stored halved and the strict upper part of a
-// block as zero, so the symmetric product treats every stored element alike:
-// A(i, j) feeds row i (A v_j) and column j (conj(A) v_i).  A grid whose blocks
+// dim <= 192.  Blocks below the 32 x 32 diagonal super-blocks hold A(i, j);
+// the two blocks of a row block that lie in its diagonal super-block hold
+// A(i, j) / 2 for every element, above the diagonal too, so that each pair
+// (i, j), (j, i) of a super-block is stored twice at half weight.  The
+// symmetric product then treats every stored element alike: A(i, j) feeds
+// row i (A v_j) and column j (conj(A) v_i).  A grid whose blocks
 // do not all fit its register budget keeps its last NL blocks in LDS instead,
 // in lane-private slots (slot-major, so a wave's accesses are consecutive).
 //
@@ -138,14 +141,12 @@ struct RegLayout {
     static constexpr int base(int l) { return l * (l + 1); }  // sum of mmax(q) + 1 over q < l
     static constexpr int count() { return base(LR); }
     static constexpr int idx(int l, int m) { return base(l) + m; }
-    // first block column of row block l that may hold i <= j for some lane
-    static constexpr int mcross(int l) {
-        const int t = PR * l - PC + 1;  // smallest m with PR*l - PC*m <= PC - 1
-        return t <= 0 ? 0 : (t + PC - 1) / PC;
-    }
-    static constexpr int XC = PR / PC + 1;  // crossing blocks per row block (upper bound)
     // first row block l with block (l, m) stored: the smallest l with 2l + 1 >= m
     static constexpr int lmin(int m) { return m / 2; }
+    // blocks (l, 2l) and (l, 2l + 1) lie in the diagonal 32 x 32 super-block of
+    // rows 32l..32l+31: i and j there are in the same super-block, and so are
+    // the mirror pair (j, i)
+    static constexpr bool straddles(int l, int m) { return m >= 2 * l; }
 };
 
 }  // namespace
@@ -173,7 +174,6 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     using Lay = RegLayout<PR, PC, LR, LC>;
     using f2 = c2<R>;
     constexpr int NS = Lay::count();
-    constexpr int XC = Lay::XC;
     constexpr int NX = PR * LR;  // rows covered by the grid
     constexpr int NXR = (NX + 63) / 64;
     static_assert(PC == 16 && BS % 64 == 0, "row groups are the 16-lane DPP rows of a wave");
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
     VT* tau_o = tauout + kk * de_kstride;
     const int s_stop = s_end < n - 1 ? s_end : n - 1;
 
-    // ---- load the lower triangle (diagonal halved, strict upper zero) ----
+    // ---- load: below the diagonal super-blocks A(i, j), inside them A(i, j) / 2 ----
     f2 A[NR];
     // slot i of this lane (i is a compile-time constant wherever these are used)
     auto aget = [&](int i) -> f2 {
@@ -226,26 +226,20 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         for (int m = 0; m <= Lay::mmax(l); ++m) {
             const int i = r + PR * l, j = c + PC * m;
             f2 a = mk<R>(0, 0);
-            if (i < n && j < n && i >= j) {
-                // a wave's 16 columns j are consecutive: IN 1 and 2 read them as
-                // contiguous runs (row i of the stored triangle)
-                const THin h = H[IN == 0 ? i + (int64_t)j * ld : j + (int64_t)i * ld];
-                const R f = (i == j) ? R(0.5) : R(1);
-                a = mk<R>((R)h.x * f, (IN == 1 ? -(R)h.y : (R)h.y) * f);
+            if (i < n && j < n && (i >= j || Lay::straddles(l, m))) {
+                // the lower element (p, q), p >= q: (i, j), or (j, i) conjugated above the
+                // diagonal.  A wave's 16 columns j are consecutive: IN 1 and 2 read the
+                // lower elements as contiguous runs (row i of the stored triangle)
+                const bool up = i < j;
+                const int p = up ? j : i, q = up ? i : j;
+                const THin h = H[IN == 0 ? p + (int64_t)q * ld : q + (int64_t)p * ld];
+                const R f = Lay::straddles(l, m) ? R(0.5) : R(1);
+                const R hy = (IN == 1) != up ? -(R)h.y : (R)h.y;
+                a = mk<R>((R)h.x * f, hy * f);
             }
             aset(Lay::idx(l, m), a);
         }
     }
-    // update weights of the blocks that straddle the diagonal: 1 below, 1/2 on, 0 above
-    R HF[LOW ? 1 : LR][LOW ? 1 : XC];  // LOW recomputes the weights where used
-#pragma unroll
-    for (int l = 0; l < LR; ++l)
-#pragma unroll
-        for (int q = 0; q < XC; ++q) {
-            const int m = Lay::mcross(l) + q;
-            const int dd = (r - c) + PR * l - PC * m;
-            if constexpr (!LOW) HF[l][q] = dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0));
-        }
     const f2 one = mk<R>(1, 0), zero = mk<R>(0, 0);
 
     for (int s = 0; s < s_stop; ++s) {
@@ -263,7 +257,8 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                         const int i = r + PR * l;
                         if (m <= Lay::mmax(l)) {
                             const f2 a = aget(Lay::idx(l, m));
-                            X[i] = sel(i > s, a, zero);
+                            const R f = Lay::straddles(l, m) ? R(2) : R(1);
+                            X[i] = sel(i > s, mk(f * a.x, f * a.y), zero);
                             XD[i] = a;
                         } else {
                             X[i] = zero;  // block above the diagonal: rows < s
@@ -389,7 +384,11 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
         qv = wave::sum(qv);
         if (lane == 0) Q[wv] = qv;
         __syncthreads();
-        // ---- d. y = row sums + column sums of the waves, one row per lane ----
+        // ---- d. y = row sums + column sums of the waves, one row per lane; then
+        // w = tau y + alpha v, alpha = -(tau/2) p^H v = -(1/2)|tau|^2 v^H A v, and v
+        // itself, published for the update: w over y in Y, v over the row sums in RP
+        // (the next writes of both come after a barrier that every wave passes only
+        // once it has finished this step) ----
         if (tid < NX) {
             f2 y = zero;
             if (tid > s && tid < n) {
@@ -397,49 +396,36 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
 #pragma unroll
                 for (int w = 0; w < NW; ++w) y = add(y, CP[w][tid]);
             }
-            Y[tid] = y;
+            R vav = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) vav += Q[w];
+            const R alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
+            const f2 v = sel(tid == s + 1, one, mul(X[tid], sc));
+            f2 wt = mul(tau, y);
+            wt.x = fma(alpha, v.x, wt.x);
+            wt.y = fma(alpha, v.y, wt.y);
+            Y[tid] = wt;
+            RP[tid] = v;
         }
         __syncthreads();
-        // ---- e. w = tau y + alpha v, alpha = -(tau/2) p^H v = -(1/2)|tau|^2 v^H A v; rank-2 update ----
-        R vav = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) vav += Q[w];
-        const R alpha = -(tau.x * tau.x + tau.y * tau.y) * vav;  // v^H A v = 2 vav
-        auto wcol = [&](int m, f2 v) -> f2 {
-            f2 w = mul(tau, Y[c + PC * m]);
-            w.x = fma(alpha, v.x, w.x);
-            w.y = fma(alpha, v.y, w.y);
-            return w;
-        };
-        f2 wr[LR], wc[LOW ? 1 : LC];
+        // ---- e. rank-2 update A -= v w^H + w v^H, at half weight in the diagonal super-blocks ----
+        f2 wr[LR], vh[LR], wh[LR];
 #pragma unroll
         for (int l = 0; l < LR; ++l) {
-            f2 w = mul(tau, Y[r + PR * l]);
-            w.x = fma(alpha, vr[l].x, w.x);
-            w.y = fma(alpha, vr[l].y, w.y);
-            wr[l] = w;
-        }
-        if constexpr (!LOW) {
-#pragma unroll
-            for (int m = 0; m < LC; ++m) wc[m] = wcol(m, vc[m]);
+            wr[l] = Y[r + PR * l];
+            vh[l] = mk(R(0.5) * vr[l].x, R(0.5) * vr[l].y);
+            wh[l] = mk(R(0.5) * wr[l].x, R(0.5) * wr[l].y);
         }
 #pragma unroll
         for (int m = LC - 1; m >= 0; --m) {
             if (m >= m0) {
-                const f2 vcm = LOW ? vcol(m) : vc[LOW ? 0 : m];
-                const f2 wcm = LOW ? wcol(m, vcm) : wc[LOW ? 0 : m];
+                const f2 vcm = LOW ? RP[c + PC * m] : vc[LOW ? 0 : m];
+                const f2 wcm = Y[c + PC * m];
 #pragma unroll
                 for (int l = Lay::lmin(m); l < LR; ++l) {
-                    // a -= v_i conj(w_j) + w_i conj(v_j), scaled by 1, 1/2 or 0 on blocks that
-                    // straddle the diagonal
-                    f2 vi = vr[l], wi = wr[l];
-                    if (m >= Lay::mcross(l)) {
-                        const int dd = (r - c) + PR * l - PC * m;
-                        const R h = LOW ? (dd > 0 ? R(1) : (dd == 0 ? R(0.5) : R(0)))
-                                        : HF[LOW ? 0 : l][LOW ? 0 : m - Lay::mcross(l)];
-                        vi = mk(h * vi.x, h * vi.y);
-                        wi = mk(h * wi.x, h * wi.y);
-                    }
+                    // a -= v_i conj(w_j) + w_i conj(v_j)
+                    const bool half = Lay::straddles(l, m);
+                    const f2 vi = half ? vh[l] : vr[l], wi = half ? wh[l] : wr[l];
                     f2 a = aget(Lay::idx(l, m));
                     a.x = fma(-vi.x, wcm.x, a.x);
                     a.x = fma(-vi.y, wcm.y, a.x);
@@ -464,7 +450,7 @@ __global__ __launch_bounds__(PR* PC, MINWG) void tridiag_reg_kernel(const THin* 
                 const int i = r + PR * l, j = c + PC * m;
                 if (i < n && j >= s_stop && i >= j) {
                     const f2 a = aget(Lay::idx(l, m));
-                    const R f = (i == j) ? R(2) : R(1);
+                    const R f = Lay::straddles(l, m) ? R(2) : R(1);
                     B[(j - s_stop) + (int64_t)(i - s_stop) * ldb] = store_c<VT>(a.x * f, a.y * f);
                 }
             }

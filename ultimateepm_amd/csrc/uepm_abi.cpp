@@ -175,6 +175,12 @@ void plan_apply_config(uepm_plan* p) {
     p->tp = uepm::tri_plan(p->dim, (int)sizeof(double2), p->ctx->lds_limit, bs);
     p->tp32 = uepm::tri_plan(p->dim, (int)sizeof(float2), p->ctx->lds_limit, bs);
     p->bt_bands = p->cfg.bt_bands > 0 ? p->cfg.bt_bands : 16;
+    const int nbr = mixed ? std::min(p->dim, p->nb + kMixedExtra) : p->nb;
+    // automatic: the blocked MFMA back-transform above dim 192 or past 16 Ritz vectors,
+    // where it is 2.5-3x faster; below, the reflector-by-reflector kernel (no LDS)
+    // shares the chip better with the other lane's register grids
+    p->bt_wy = mixed && p->cfg.bt_bands == 0 && (p->dim > 192 || nbr > 16) &&
+               uepm::apply_q_wy_fits(p->dim, nbr, p->ctx->lds_limit);
 }
 
 }  // namespace uepm_host
@@ -300,7 +306,8 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
                 HIP_OK(uepm::launch_cluster_orth(w.d, w.e, w.Ev,
                                                  w.Z, n, nbr, c, cs));
                 HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf, w.Z, nbr,
-                                                w.X0, p->P, kc, p->G.as<int32_t>(), c, p->bt_bands, cs));
+                                                w.X0, p->P, kc, p->G.as<int32_t>(), c,
+                                                p->bt_wy ? 0 : p->bt_bands, cs));
                 HIP_OK(rec(ci, kBack, 1));
                 HIP_OK(rec(ci, kRefine, 0));
                 HIP_OK(uepm::launch_refine(Hc, n, n, w.X0, nbr, nb, devals + c0 * nb,
@@ -717,7 +724,11 @@ int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, ue
     std::vector<uepm_config> cand;
     const bool mixed = p->flags & UEPM_MIXED;
     const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
-    for (int bt : {8, 16}) {
+    // back-transform: the blocked MFMA kernel (0, precision form) or 8 / 16 bands per
+    // workgroup of the reflector-by-reflector one
+    std::vector<int> bts = {8, 16};
+    if (mixed) bts.insert(bts.begin(), 0);
+    for (int bt : bts) {
         if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk, p->cfg.lanes});
         for (int tb : {256, 512}) {
             cand.push_back({1, tb, bt, p->cfg.max_chunk, p->cfg.lanes});

@@ -59,7 +59,9 @@ constexpr int kMixedExtra = 4;
 constexpr double kMixedBisectTol = 1e-10;
 // Inverse-iteration sweeps: the FP64 form's vectors are its output; the
 // precision form's only seed the FP64 Rayleigh-Ritz step, whose values are
-// second order in their error (two sweeps leave ~1e-5 in the vectors)
+// second order in their error (two sweeps leave ~1e-5 in the vectors; one
+// sweep leaves some k-points with two vectors of a close pair nearly parallel,
+// and the Ritz step then misses a band by tens of meV)
 constexpr int kFp64InvSweeps = 3;
 constexpr int kMixedInvSweeps = 2;
 
@@ -100,8 +102,9 @@ struct uepm_plan {
     hipEvent_t lane_done[2] = {nullptr, nullptr};
     uepm::TriPlan tp{}, tp32{};
     uepm_config cfg{};    // requested configuration (0 fields = automatic)
-    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 256)
-    int bt_bands = 16;    // bands per back-transform workgroup
+    bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 288)
+    int bt_bands = 16;    // bands per workgroup of the reflector-by-reflector back-transform
+    bool bt_wy = false;   // precision form: the blocked MFMA back-transform instead
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
     const char* stage_name[uepm_host::kNumStages] = {"assemble_kernel", "", "", "", ""};  // kernels of the last solve
