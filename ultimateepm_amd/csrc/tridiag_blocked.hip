@@ -93,7 +93,7 @@ __host__ __device__ inline int padded_len(int L, int MOD) { return L + (((2 - L)
 
 constexpr int kTriNBP = 4;
 constexpr int kTriParts = 4;
-constexpr int kTriMaxRowsPerLane = 9;  // n <= 576  // max column parts per row in the symv
+constexpr int kTriMaxRowsPerLane = 9;  // rows per lane in the symmetric product: n <= 576
 
 template <typename T>
 struct TriSmem {
@@ -103,7 +103,7 @@ struct TriSmem {
         int b = ((n * 4 + 15) / 16) * 16;  // colofs
         b += 2 * NBP * n * E;              // panel V and W columns
         b += 2 * n * E;                    // y (row sums / w), ycol (column sums of global rows)
-        b += kTriParts * n * E;            // per-part partial row sums of the symv
+        b += kTriParts * n * E;            // partial row sums of the symv (waves w, w + 4 share one)
         b += 2 * NBP * E;                  // dots a_q, b_q
         b += NBP * E;                      // alpha of the panel's steps
         b += 16 * 8;                       // scalars
@@ -285,48 +285,93 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                 e[j] = beta;
                 tau[j] = tj;
             }
-            // S3a: y = A v over row blocks (lane = row).  Blocks are split at c0 so
-            // a wave is either all-LDS-rows or all-global-rows; LDS row blocks are
-            // further split into T_ column parts so that idle waves take a share.
-            // Walks are software-pipelined four elements deep.
-            int T_ = 1;
+            // S3: y = A v.  Every wave keeps partial row sums for rows r = 64 q + lane
+            // in yq[q] (a lane's rows are fixed, whatever the column).
+            //  G. Global columns c in [j+1, c0), one wave per column, round robin: the
+            //     coalesced column A(r, c), r >= c, feeds the row part y_r += A(r, c) v_c,
+            //     and its column part y_c = sum_{r > c} conj(A(r, c)) v_r is one wave
+            //     reduction (every element right of the diagonal in row c sits in
+            //     column c, so ycol[c] is complete).  Loads run PF - 1 columns ahead
+            //     (three in complex64, one in complex128, as registers allow).
+            //  L. Rows r >= rl0 = max(c0, j+1) against the LDS columns [rl0, n), both
+            //     triangles (A(r, c), or conj(A(c, r)) above the diagonal), in items of
+            //     one 64-row block and one column part; nq * TL items, a multiple of NW.
+            //  The wave partials go to Yp: waves w < NP write Yp[w], then waves w >= NP
+            //  add theirs to Yp[w - NP] (a fixed order, so the sums are reproducible).
             {
-                const int slow0 = j + 1, slow1 = max(j + 1, min(c0, n));  // global rows [slow0, slow1)
-                const int ns = (slow1 - slow0 + 63) >> 6;
-                const int nf = (n - slow1 + 63) >> 6;
-                T_ = nf > 0 ? max(1, min(kTriParts, (NW - ns) / nf)) : 1;
-                for (int blk = wv; blk < ns + nf * T_; blk += NW) {
-                    if (blk >= ns) {
-                        // rows r >= c0, column part pp of [j+1, n)
-                        const int fb = (blk - ns) % nf, pp = (blk - ns) / nf;
-                        const int rmin = slow1 + 64 * fb;
-                        const int r = min(rmin + lane, n - 1);
-                        const bool live = rmin + lane < n;
-                        const int cs = j + 1 + (pp * m) / T_, ce = j + 1 + ((pp + 1) * m) / T_;
+                constexpr int NQ = kTriMaxRowsPerLane;
+                constexpr int PF = sizeof(T) == 4 ? 4 : 2;
+                const int cg1 = min(c0, n);
+                const int rl0 = max(c0, j + 1);
+                // v at this lane's rows: kept in registers in complex64, re-read from LDS
+                // in complex128 (whose load buffers take the registers)
+                constexpr bool VREG = sizeof(T) == 4;
+                cx<T> yq[NQ], vq[VREG ? NQ : 1];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    yq[q] = zero;
+                    const int r = 64 * q + lane;
+                    if constexpr (VREG) vq[q] = (64 * q < n && r < n) ? vj[r] : zero;
+                }
+                // ---- G ----
+                auto gload = [&](int c, cx<T>(&b)[NQ]) {
+                    const cx<T>* col = gcol(c);
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int r = 64 * q + lane;
+                        b[q] = zero;
+                        if (64 * q + 63 >= c && 64 * q < n)  // wave-uniform
+                            if (r >= c && r < n) b[q] = col[r];
+                    }
+                };
+                auto gdo = [&](int c, const cx<T>(&b)[NQ]) {
+                    const cx<T> vc = vj[c];
+                    cx<T> cs = zero;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        if (64 * q + 63 >= c && 64 * q < n) {
+                            const int r = 64 * q + lane;
+                            cfma(yq[q], b[q], vc);  // b is zero outside rows [c, n)
+                            cfmac(cs, r > c ? b[q] : zero, VREG ? vq[VREG ? q : 0] : vj[min(r, n - 1)]);
+                        }
+                    }
+                    cs.x = wsum(cs.x);
+                    cs.y = wsum(cs.y);
+                    if (lane == 0) ycol[c] = cs;
+                };
+                {
+                    cx<T> bb[PF][NQ];
+#pragma unroll
+                    for (int p = 0; p + 1 < PF; ++p) {
+                        const int cc = j + 1 + wv + p * NW;
+                        if (cc < cg1) gload(cc, bb[p]);
+                    }
+                    for (int c = j + 1 + wv; c < cg1; c += PF * NW) {
+#pragma unroll
+                        for (int p = 0; p < PF; ++p) {
+                            const int cc = c + p * NW;
+                            const int cn = cc + (PF - 1) * NW;  // into the slot freed last
+                            if (cn < cg1) gload(cn, bb[(p + PF - 1) % PF]);
+                            if (cc < cg1) gdo(cc, bb[p]);
+                        }
+                    }
+                }
+                // ---- L ----
+                if (rl0 < n) {
+                    const int qa = rl0 >> 6, nq = ((n + 63) >> 6) - qa;
+                    int TL = NW;
+                    while (TL > 1 && (nq * (TL >> 1)) % NW == 0) TL >>= 1;
+                    const int m = n - rl0;
+                    for (int it = wv; it < nq * TL; it += NW) {
+                        const int q = qa + it % nq, pp = it / nq;
+                        const int rr = 64 * q + lane;
+                        const bool live = rr >= rl0 && rr < n;
+                        const int r = min(max(rr, rl0), n - 1);
+                        const int cs = rl0 + (pp * m) / TL, ce = rl0 + ((pp + 1) * m) / TL;
                         cx<T> acc0 = zero, acc1 = zero;
-                        // global columns [cs, min(ce, c0)): all c < r, coalesced row walk
-                        if (cs < c0) {
-                            const int cg = min(ce, c0);
+                        if (cs < ce) {
                             int off = colofs[cs] - cs;
                             int P = padded_len(n - cs, MOD);
-                            int c = cs;
-                            for (; c + 1 < cg; c += 2) {
-                                const int off1 = off + P - 1;
-                                const int P1 = padded_len(n - c - 1, MOD);
-                                const cx<T> a0 = gmat[off + r], a1 = gmat[off1 + r];
-                                cfma(acc0, a0, vj[c]);
-                                cfma(acc1, a1, vj[c + 1]);
-                                off = off1 + P1 - 1;
-                                P = padded_len(n - c - 2, MOD);
-                            }
-                            if (c < cg) cfma(acc0, gmat[off + r], vj[c]);
-                        }
-                        // LDS columns [max(cs, c0), ce): row part A(r, c) for c <= r,
-                        // column part conj(A(c, r)) for c > r
-                        const int cl = max(cs, c0);
-                        if (cl < ce) {
-                            int off = colofs[cl] - cl;
-                            int P = padded_len(n - cl, MOD);
                             const int bcol = colofs[r] - r;
                             auto fetch = [&](int c, int o) -> cx<T> {
                                 const bool in = c <= r;
@@ -334,7 +379,7 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                                 a.y = in ? a.y : -a.y;
                                 return a;
                             };
-                            int c = cl;
+                            int c = cs;
                             if (c + 3 < ce) {
                                 int o0 = off, o1, o2, o3;
                                 o1 = o0 + P - 1; P = padded_len(n - c - 1, MOD);
@@ -371,48 +416,31 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                                 P = padded_len(n - c - 1, MOD);
                             }
                         }
-                        if (live) Yp[pp * n + r] = acc0 + acc1;
-                    } else {
-                        // rows r in [j+1, c0): row part only (global columns c in [j+1, r]);
-                        // the column part comes from S3b.
-                        const int rmin = slow0 + 64 * blk;
-                        const int rr = rmin + lane;
-                        const bool live = rr < slow1;
-                        const int r = min(rr, slow1 - 1);
-                        const int cend = min(slow1, rmin + 64);
-                        cx<T> acc0 = zero;
-                        int off = colofs[j + 1] - (j + 1);
-                        int P = padded_len(n - j - 1, MOD);
-                        for (int c = j + 1; c < cend; ++c) {
-                            cx<T> a = gmat[max(off + r, 0)];  // above-diagonal reads are discarded
-                            if (c > r) a = zero;
-                            cfma(acc0, a, vj[c]);
-                            off += P - 1;
-                            P = padded_len(n - c - 1, MOD);
+                        const cx<T> acc = live ? acc0 + acc1 : zero;
+#pragma unroll
+                        for (int qq = 0; qq < NQ; ++qq)
+                            if (qq == q) yq[qq] = yq[qq] + acc;  // wave-uniform
+                    }
+                }
+                // ---- wave partials to Yp ----
+                constexpr int NP = NW < kTriParts ? NW : kTriParts;
+                static_assert(NW <= 2 * kTriParts, "two rounds of partial sums");
+                if (wv < NP) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) {
+                        const int r = 64 * q + lane;
+                        if (64 * q < n && r < n) Yp[wv * n + r] = yq[q];
+                    }
+                }
+                if constexpr (NW > NP) {
+                    __syncthreads();
+                    if (wv >= NP) {
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            const int r = 64 * q + lane;
+                            if (64 * q < n && r < n) Yp[(wv - NP) * n + r] = Yp[(wv - NP) * n + r] + yq[q];
                         }
-                        if (live) Yp[r] = acc0;
                     }
-                }
-            }
-            // S3b: column part of rows r in [j+1, c0): wave dots over global column r,
-            // eight rows per wave at a time with all loads issued before the reductions.
-            for (int r0 = j + 1 + wv; r0 < min(c0, n); r0 += 8 * NW) {
-                cx<T> accr[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int r = r0 + i * NW;
-                    cx<T> acc = zero;
-                    if (r < min(c0, n)) {
-                        const cx<T>* colr = gcol(r);
-                        for (int c = r + 1 + lane; c < n; c += 64) cfmac(acc, colr[c], vj[c]);
-                    }
-                    accr[i] = acc;
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int r = r0 + i * NW;
-                    const T sx = wsum(accr[i].x), sy = wsum(accr[i].y);
-                    if (lane == 0 && r < min(c0, n)) ycol[r] = cxm<T>(sx, sy);
                 }
             }
             // S3c: a_q = sum_r conj(W(r,q)) v_r, b_q = sum_r conj(V(r,q)) v_r, r in [j+1, n)
@@ -438,9 +466,9 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                 cx<T> w = zero;
                 if (r > j) {
                     cx<T> y = Yp[r];
+#pragma unroll
+                    for (int pp = 1; pp < (NW < kTriParts ? NW : kTriParts); ++pp) y = y + Yp[pp * n + r];
                     if (r < c0) y = y + ycol[r];
-                    else
-                        for (int pp = 1; pp < T_; ++pp) y = y + Yp[pp * n + r];
                     for (int q = 0; q < jj; ++q) {
                         const cx<T> aq = alph[q], bq = dots[NBP + q];
                         const cx<T> cv = dots[q] + cxm<T>(2 * aq.x, 0) * bq;
@@ -511,19 +539,12 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                     xre[u + U] = -L.y;
                     xim[u + U] = L.x;
                 }
-                for (int bj = 0; bj <= bi; ++bj) {
+                // tile (bi, bj): accumulators are loaded ahead, so the global (or LDS)
+                // latency of the next tiles overlaps this tile's MFMAs
+                const int rr0 = p1 + 16 * bi;
+                auto cload = [&](int bj, v4& cre, v4& cim) {
                     const int cc = p1 + 16 * bj + li;
-                    T y[2 * U];
-#pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const cx<T> R = Rval(cc, lq + 4 * u);
-                        y[u] = R.x;
-                        y[u + U] = R.y;
-                    }
-                    const int rr0 = p1 + 16 * bi;
-                    v4 cre, cim;
-                    cx<T>* cbase = nullptr;
-                    if (cc < n) cbase = cc < c0 ? gcol(cc) : lcol(cc);
+                    cx<T>* cbase = cc < n ? (cc < c0 ? gcol(cc) : lcol(cc)) : nullptr;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int rr = rr0 + M::row(lq, v);
@@ -532,15 +553,53 @@ __global__ __launch_bounds__(BS, 1) void tridiag_blocked_kernel(const double2* _
                         cre[v] = a.x;
                         cim[v] = a.y;
                     }
+                };
+                auto tile = [&](int bj, v4 cre, v4 cim) {
+                    const int cc = p1 + 16 * bj + li;
+                    T y[2 * U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const cx<T> R = Rval(cc, lq + 4 * u);
+                        y[u] = R.x;
+                        y[u + U] = R.y;
+                    }
 #pragma unroll
                     for (int q = 0; q < 2 * U; ++q) {
                         cre = M::op(xre[q], y[q], cre);
                         cim = M::op(xim[q], y[q], cim);
                     }
+                    cx<T>* cbase = cc < n ? (cc < c0 ? gcol(cc) : lcol(cc)) : nullptr;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int rr = rr0 + M::row(lq, v);
                         if (cc < n && rr < n && rr >= cc) cbase[rr] = cxm<T>(cre[v], rr == cc ? (T)0 : cim[v]);
+                    }
+                };
+                if constexpr (sizeof(T) == 4) {
+                    // two tiles in flight: slot 0 holds tile bj, slot 1 tile bj + 1
+                    v4 re0, im0, re1, im1;
+                    cload(0, re0, im0);
+                    if (bi >= 1) cload(1, re1, im1);
+                    for (int bj = 0; bj <= bi; bj += 2) {
+                        {
+                            const v4 cre = re0, cim = im0;
+                            if (bj + 2 <= bi) cload(bj + 2, re0, im0);
+                            tile(bj, cre, cim);
+                        }
+                        if (bj + 1 <= bi) {
+                            const v4 cre = re1, cim = im1;
+                            if (bj + 3 <= bi) cload(bj + 3, re1, im1);
+                            tile(bj + 1, cre, cim);
+                        }
+                    }
+                } else {
+                    // complex128: one tile ahead (a second slot would spill)
+                    v4 nre, nim;
+                    cload(0, nre, nim);
+                    for (int bj = 0; bj <= bi; ++bj) {
+                        const v4 cre = nre, cim = nim;
+                        if (bj < bi) cload(bj + 1, nre, nim);
+                        tile(bj, cre, cim);
                     }
                 }
             }
@@ -606,6 +665,7 @@ template <typename T>
 static hipError_t launch_tri_blocked_t(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                        void* Vout, double* dd, double* de, void* dtau, int s_end, hipStream_t s) {
     if (s_end < tp.n - 1 && (s_end < 0 || s_end % kTriNBP != 0)) return hipErrorInvalidValue;
+    if (tp.n > 64 * kTriMaxRowsPerLane) return hipErrorInvalidValue;  // rows per lane in the product
     if (tp.bs >= 512) return launch_tri_blocked_bs<T, 512>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
     return launch_tri_blocked_bs<T, 256>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
 }
