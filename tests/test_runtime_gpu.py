@@ -96,10 +96,12 @@ def test_tuned_plan_keeps_results(gpu_ctx):
     p.close()
 
 
-@pytest.mark.parametrize("sym,n,soc,nb", [("Si", 14, False, 12), ("Ge", 10, True, 16), ("Ge", 11, True, 24)])
+@pytest.mark.parametrize("sym,n,soc,nb", [("Si", 14, False, 12), ("Ge", 10, True, 16), ("Ge", 11, True, 24),
+                                          ("Ge", 14, True, 16)])
 def test_wy_back_transform_matches_per_reflector(gpu_ctx, sym, n, soc, nb):
     """Precision form above dim 192: the compact-WY MFMA back-transform (automatic,
-    bt_bands 0; one band tile at 16 Ritz vectors, two at 20 and 28) and the
+    bt_bands 0; one band tile at 16 Ritz vectors, two at 20 and 28, and at dim 518
+    two one-tile passes, since two tiles of X do not fit the LDS there) and the
     reflector-by-reflector kernel (bt_bands 16) give the same eigenvalues and
     eigenvectors to the precision form's Ritz step."""
     print(f"case: WY vs per-reflector back-transform, {sym} n={n} soc={soc} nb={nb}", flush=True)

@@ -270,7 +270,8 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
 // above the diagonal (the LAPACK xLARFT forward/columnwise recurrence
 // T_{0:j,j} = -tau_j T_{0:j,0:j} G_{0:j,j}, T_jj = tau_j, inverted), so
 // W' = T W is a back substitution with U.  One 256-lane workgroup per
-// k-point; X (n rows x 16 NT bands, complex64) lives in LDS.
+// k-point; X (n rows x 16 NT bands, complex64: bands b0 .. b0 + nbp - 1 of the
+// nb per k-point, the rest of the tile zero) lives in LDS.
 //   1. [G | W] = V^H [V | X] on v_mfma_f32_16x16x4f32 (complex products as
 //      four real ones), the rows split over the four waves; the per-wave
 //      partial sums are added in a fixed order, so results do not depend on
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(256) void apply_q_kernel(const VT* __restrict__ V, 
 template <int NT>
 __global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __restrict__ V, int n, int ld,
                                                          const float2* __restrict__ tauall,
-                                                         const double* __restrict__ Z, int nb,
+                                                         const double* __restrict__ Z, int nb, int b0, int nbp,
                                                          double2* __restrict__ X) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     constexpr int NBT = 16 * NT;
@@ -311,10 +312,10 @@ __global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __rest
         return Vk[row + (int64_t)j * ld];
     };
 
-    // X = Z (real); bands past nb are zero
+    // X = Z (real) for bands b0 .. b0 + nbp - 1 of the nb per k-point; the rest of the tile is zero
     for (int e = tid; e < n * NBT; e += 256) {
         const int b = e / n, row = e - b * n;
-        const float z = b < nb ? (float)Z[(kk * nb + b) * (int64_t)n + row] : 0.f;
+        const float z = b < nbp ? (float)Z[(kk * nb + b0 + b) * (int64_t)n + row] : 0.f;
         Xs[row * XS + b] = make_float2(z, 0.f);
     }
     __syncthreads();
@@ -451,10 +452,10 @@ __global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __rest
         __syncthreads();
     }
     // X0[k][band][row] in FP64
-    for (int e = tid; e < n * nb; e += 256) {
+    for (int e = tid; e < n * nbp; e += 256) {
         const int b = e / n, row = e - b * n;
         const float2 x = Xs[row * XS + b];
-        X[(kk * nb + b) * (int64_t)n + row] = make_double2((double)x.x, (double)x.y);
+        X[(kk * nb + b0 + b) * (int64_t)n + row] = make_double2((double)x.x, (double)x.y);
     }
 }
 
@@ -528,22 +529,27 @@ size_t wy_static_bytes(int nt) {
 }  // namespace
 
 bool apply_q_wy_fits(int n, int nb, int lds_limit) {
+    // 17..32 bands run as one two-tile pass, or as two one-tile passes when the
+    // two-tile X does not fit the LDS (each pass reads the reflectors once more)
     if (nb > 32 || n < 2) return false;
-    const int nt = nb > 16 ? 2 : 1;
-    return wy_lds_bytes(n, nt) + wy_static_bytes(nt) <= (size_t)lds_limit;
+    return wy_lds_bytes(n, 1) + wy_static_bytes(1) <= (size_t)lds_limit;
 }
 
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                              int bands_per_group, hipStream_t s) {
+                              int bands_per_group, int lds_limit, hipStream_t s) {
     if (bands_per_group == 0) {
         // the blocked MFMA back-transform
-        const int nt = nb > 16 ? 2 : 1;
+        if (!apply_q_wy_fits(n, nb, lds_limit)) return hipErrorInvalidValue;
+        const bool two = nb > 16 && wy_lds_bytes(n, 2) + wy_static_bytes(2) <= (size_t)lds_limit;
+        const int nt = two ? 2 : 1;
         const size_t shm = wy_lds_bytes(n, nt);
         auto kern = nt == 1 ? apply_q_wy_kernel<1> : apply_q_wy_kernel<2>;
         hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (err != hipSuccess) return err;
-        hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb, dX);
+        for (int b0 = 0; b0 < nb; b0 += 16 * nt)
+            hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb, b0,
+                               std::min(16 * nt, nb - b0), dX);
         return hipGetLastError();
     }
     return launch_apply<float, float2>(dV, n, ld, dtau, dZ, nb, dX, nullptr, P, dk, dG, nk, nullptr, nullptr, 0,

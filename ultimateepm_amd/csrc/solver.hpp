@@ -104,10 +104,11 @@ hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* d
                               int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
                               int bands_per_group, hipStream_t s);
 // Precision form: bands_per_group 0 selects the blocked MFMA back-transform
-// (compact WY, sixteen reflectors per block), which needs apply_q_wy_fits.
+// (compact WY, sixteen reflectors per block), which needs apply_q_wy_fits
+// under the per-workgroup LDS limit lds_limit (bytes).
 hipError_t launch_apply_q_c64(const float2* dV, int n, int ld, const float2* dtau, const double* dZ, int nb,
                               double2* dX, const DevParams& P, const double* dk, const int32_t* dG, int64_t nk,
-                              int bands_per_group, hipStream_t s);
+                              int bands_per_group, int lds_limit, hipStream_t s);
 bool apply_q_wy_fits(int n, int nb, int lds_limit);
 
 // Device-to-device copy (16-byte aligned pointers take the vector path).

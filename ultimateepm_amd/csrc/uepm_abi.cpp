@@ -307,7 +307,7 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
                                                  w.Z, n, nbr, c, cs));
                 HIP_OK(uepm::launch_apply_q_c64(Vc, n, n, w.tauf, w.Z, nbr,
                                                 w.X0, p->P, kc, p->G.as<int32_t>(), c,
-                                                p->bt_wy ? 0 : p->bt_bands, cs));
+                                                p->bt_wy ? 0 : p->bt_bands, p->ctx->lds_limit, cs));
                 HIP_OK(rec(ci, kBack, 1));
                 HIP_OK(rec(ci, kRefine, 0));
                 HIP_OK(uepm::launch_refine(Hc, n, n, w.X0, nbr, nb, devals + c0 * nb,
