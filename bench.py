@@ -235,22 +235,25 @@ def config3_leg(ctx, nk: int = 100_000, seed: int = SEED) -> dict:
     return out
 
 
-def dims_leg(ctx, seed: int = SEED) -> list:
+def dims_leg(ctx, seed: int = SEED, only=None) -> list:
     """Throughput by basis dimension: Si (n = 6..14 shells, dims 59..259) and Ge with
     spin-orbit (n = 6..14, dims 118..518), non-local, 16 bands, eigenvalues only,
     2048 seeded BZ k-points (512 above dim 200), both forms; one untimed solve,
     then one timed solve (wall clock over a device-resident batch) and the stage
-    split of a third."""
+    split of a third.  `only`: the dimensions to run (default: all)."""
     from ultimateepm_amd import api
     from ultimateepm_amd.material import load
 
     rows = []
     for sym, soc in (("Si", False), ("Ge", True)):
         for n in (6, 8, 10, 12, 14):
+            G = api.basis_vectors(n)
+            if only and len(G) * (2 if soc else 1) not in only:
+                continue
             for mode in ("fp64", "mixed"):
                 flags = api.NONLOCAL | (api.SOC if soc else 0) | (api.MIXED if mode == "mixed" else 0)
                 nb = 16
-                p = api.Plan(ctx, load(sym), api.basis_vectors(n), flags, nb)
+                p = api.Plan(ctx, load(sym), G, flags, nb)
                 nk = 2048 if p.dim <= 200 else 512
                 dk = api.DeviceArray.from_host(ctx, api.bz_sample(seed + 5, nk))
                 de = api.DeviceArray(ctx, (nk, nb))
@@ -587,6 +590,7 @@ def main() -> int:
                     help="throughput: the headline batched solve (default); serving / batching: those forms "
                          "alone; dims: throughput by basis dimension")
     ap.add_argument("--requests", type=int, default=200, help="requests of --form serving")
+    ap.add_argument("--dims", default="", help="comma list of basis dimensions for --form dims (default: all)")
     ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
     ap.add_argument("--batch-leg-nk", type=int, default=20000,
                     help="k-points per plan of the batching leg of the default run (0: skip)")
@@ -616,10 +620,11 @@ def main() -> int:
 
     if args.form == "dims":
         ctx = api.Context(0)
-        rows = dims_leg(ctx)
-        head = next(r for r in rows if r["dim"] == 181 and r["mode"] == "mixed")
+        only = {int(x) for x in args.dims.split(",") if x.strip()}
+        rows = dims_leg(ctx, only=only)
+        head = next((r for r in rows if r["dim"] == 181 and r["mode"] == "mixed"), None)
         line = {"metric": "k-pts/s by basis dimension (16 bands, eigenvalues only); value: Si n=12 (dim 181) precision form",
-                "value": head["kpts_per_s"], "unit": "k-pts/s", "n_gpus": 1, "higher_is_better": True,
+                "value": head["kpts_per_s"] if head else None, "unit": "k-pts/s", "n_gpus": 1, "higher_is_better": True,
                 "data": "synthetic", "dtype": "c64+f64",
                 "config": {"workload": "Si n=6..14 and Ge+SOC n=6..14, non-local, nb 16, seeded BZ k-points",
                            "form": "dims"},

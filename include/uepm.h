@@ -255,11 +255,15 @@ int uepm_batch_destroy(uepm_batch* batch);
 /* ---- serving ----------------------------------------------------------------
  * Continuous batching of band-structure requests on one device: a scheduler
  * thread admits queued requests in arrival order while an in-flight slot is
- * free and the plan workspaces fit `hbm_budget_bytes`; every admitted request
- * runs on its own stream, so requests overlap on the device.  Plans are pooled
- * per (material, basis, flags, bands) with resident workspaces; idle plans are
- * evicted oldest first when the budget needs room.  A request that cannot fit
- * even alone fails with UEPM_ERR_NO_MEMORY.  Eigenvalues only (no vectors).
+ * free; every admitted request runs on its slot's stream, so requests overlap
+ * on the device.  Each of the max_inflight slots owns a workspace arena of
+ * hbm_budget_bytes / max_inflight bytes, allocated on its first request and
+ * kept; a request runs in chunks whose workspace fits it, so a running server
+ * neither allocates nor frees device memory per request.  Plans are pooled per
+ * (material, basis, flags, bands) and hold no workspace; idle plans are
+ * destroyed oldest first only past a pool of 512 (stats: evictions).  A request
+ * one of whose k-points needs more workspace than a slot's share fails with
+ * UEPM_ERR_NO_MEMORY.  used_bytes: the arenas allocated.  Eigenvalues only.
  * A request with a non-finite k-point is refused at submission
  * (UEPM_ERR_INVALID_ARG); one whose eigen-solve fails for some k-point ends
  * with UEPM_ERR_NOT_CONVERGED, its other eigenvalues returned.

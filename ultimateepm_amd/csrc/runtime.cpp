@@ -8,12 +8,15 @@
 //
 // Server (uepm_server_*): continuous batching of band-structure requests.  A
 // scheduler thread admits queued requests in arrival order whenever an
-// in-flight slot is free and the device memory budget holds the request's
-// workspace; each admitted request runs on its own stream (copy in, solve,
-// copy out, completion event), so requests that arrive while others run
-// overlap on the device.  Plans are pooled per (material, basis, flags, bands)
-// with their workspaces resident; idle plans are evicted (oldest first) when
-// the budget needs room.
+// in-flight slot is free; each admitted request runs on its slot's stream
+// (copy in, solve, copy out, completion event), so requests that arrive while
+// others run overlap on the device.  Each slot owns one workspace arena of
+// budget / max_inflight bytes, allocated on the slot's first request and kept:
+// a request's chunk workspace is carved out of it, chunks sized to fit.  Plans
+// are pooled per (material, basis, flags, bands) and hold no workspace, so a
+// request never allocates or frees device memory while others run (either
+// would synchronise the device); idle plans are destroyed, oldest first, only
+// when the pool exceeds kMaxPooledPlans.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -189,10 +192,9 @@ struct PlanKeyHash {
 struct PooledPlan {
     uepm_plan* plan = nullptr;
     PlanKey key;
-    size_t reserved = 0;   // workspace bytes held
-    int64_t reserved_nk = 0;
     Clock::time_point last_used;
 };
+constexpr size_t kMaxPooledPlans = 512;  // a plan without workspace holds a few KB of device memory
 
 struct Request {
     uint64_t id = 0;
@@ -211,6 +213,7 @@ struct Request {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    uepm_host::DeviceBuffer arena;  // the slot's chunk workspace (budget / max_inflight bytes)
     uepm_host::DeviceBuffer dk, dev;
     double* hk = nullptr;   // pinned staging
     double* hev = nullptr;
@@ -253,72 +256,55 @@ int grow_pinned(double*& p, size_t& cap, size_t need) {
     return UEPM_OK;
 }
 
-// An idle plan for the request's key, reserved for nk points, within the budget
-// (evicting idle plans, oldest first, when needed).  Returns nullptr with
-// *status set when it cannot be had now (UEPM_OK: wait) or ever (an error).
+// Workspace bytes of one slot's arena: an equal share of the budget.
+uint64_t slot_share(const uepm_server* S) { return S->budget / (uint64_t)std::max(1, S->max_inflight); }
+
+// An idle plan for the request's key (created on demand).  Returns nullptr with
+// *status set when the request can never run: its plan fails, or one k-point's
+// workspace exceeds a slot's share of the budget (UEPM_ERR_NO_MEMORY).
 PooledPlan* acquire_plan(uepm_server* S, const Request& r, const PlanKey& key, int* status) {
     *status = UEPM_OK;
-    PooledPlan* pp = nullptr;
     for (auto& q : S->plans)
-        if (!S->busy[&q] && q.key == key && (!pp || q.reserved_nk > pp->reserved_nk)) pp = &q;
-    if (!pp) {
-        int count = 0;
-        if (uepm_basis_vectors(r.shells, nullptr, 0, &count) != UEPM_OK) {
-            *status = UEPM_ERR_RUNTIME;
-            return nullptr;
-        }
-        std::vector<int32_t> G(3 * (size_t)count);
-        uepm_basis_vectors(r.shells, G.data(), count, &count);
-        uepm_plan* plan = nullptr;
-        const int st = uepm_plan_create(S->ctx, &r.mat, G.data(), count, r.flags, r.nb, &plan);
-        if (st != UEPM_OK) {
-            *status = st;
-            return nullptr;
-        }
-        // keep one request's workspace within the per-slot share of the budget
-        const size_t per_k = std::max<size_t>(1, plan_workspace_bytes(plan, 1));
-        const uint64_t share = std::max<uint64_t>(S->budget / (uint64_t)std::max(1, S->max_inflight), per_k);
-        uepm_config cfg{};
-        cfg.max_chunk = (int32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, share / per_k));
-        cfg.lanes = 1;  // the slots are the concurrency; one request keeps one workspace
-        uepm_plan_set_config(plan, &cfg);
-        S->plans.push_back(PooledPlan{plan, key, 0, 0, Clock::now()});
-        pp = &S->plans.back();
-        S->busy[pp] = false;
+        if (!S->busy[&q] && q.key == key) return &q;
+    int count = 0;
+    if (uepm_basis_vectors(r.shells, nullptr, 0, &count) != UEPM_OK) {
+        *status = UEPM_ERR_RUNTIME;
+        return nullptr;
     }
-    const int64_t chunk = plan_chunk(pp->plan, r.nk);
-    const size_t need = plan_workspace_bytes(pp->plan, chunk);
-    const size_t delta = need > pp->reserved ? need - pp->reserved : 0;
-    while (S->used + delta > S->budget) {
+    std::vector<int32_t> G(3 * (size_t)count);
+    uepm_basis_vectors(r.shells, G.data(), count, &count);
+    uepm_plan* plan = nullptr;
+    const int st = uepm_plan_create(S->ctx, &r.mat, G.data(), count, r.flags, r.nb, &plan);
+    if (st != UEPM_OK) {
+        *status = st;
+        return nullptr;
+    }
+    const size_t per_k = std::max<size_t>(1, plan_carve(plan, 1, nullptr, nullptr));
+    const uint64_t share = slot_share(S);
+    if (per_k > share) {
+        uepm_plan_destroy(plan);
+        *status = fail(UEPM_ERR_NO_MEMORY, "server: one k-point of this request needs " + std::to_string(per_k) +
+                                               " bytes of workspace, more than a slot's share (" +
+                                               std::to_string(share) + " bytes) of the budget");
+        return nullptr;
+    }
+    uepm_config cfg{};
+    cfg.max_chunk = (int32_t)std::max<uint64_t>(1, std::min<uint64_t>(4096, share / per_k));
+    cfg.lanes = 1;  // the slots are the concurrency; one request, one chunk in flight
+    uepm_plan_set_config(plan, &cfg);
+    while (S->plans.size() >= kMaxPooledPlans) {
         PooledPlan* victim = nullptr;
         for (auto& q : S->plans)
-            if (&q != pp && !S->busy[&q] && q.reserved > 0 && (!victim || q.last_used < victim->last_used)) victim = &q;
+            if (!S->busy[&q] && (!victim || q.last_used < victim->last_used)) victim = &q;
         if (!victim) break;
-        S->used -= victim->reserved;
         uepm_plan_destroy(victim->plan);
         S->busy.erase(victim);
         S->plans.remove_if([&](const PooledPlan& q) { return &q == victim; });
         ++S->stats.evictions;
     }
-    if (S->used + delta > S->budget) {
-        bool any_busy = false;
-        for (auto& kv : S->busy) any_busy |= kv.second;
-        if (!any_busy) {  // it would never fit
-            *status = UEPM_ERR_NO_MEMORY;
-            return nullptr;
-        }
-        return nullptr;  // wait for running requests to finish
-    }
-    if (delta > 0) {
-        const int st = plan_reserve(pp->plan, chunk);
-        if (st != UEPM_OK) {
-            *status = st;
-            return nullptr;
-        }
-        S->used += delta;
-        pp->reserved = need;
-    }
-    pp->reserved_nk = std::max(pp->reserved_nk, chunk);
+    S->plans.push_back(PooledPlan{plan, key, Clock::now()});
+    PooledPlan* pp = &S->plans.back();
+    S->busy[pp] = false;
     return pp;
 }
 
@@ -401,12 +387,22 @@ void worker_main(uepm_server* S) {
             }
             S->busy[pp] = true;
             sl.plan = pp;
+            // request buffers: sized for 4096 k-points and 32 bands at least, so that they
+            // rarely grow (a device reallocation would synchronise the device)
             const size_t kcount = 3 * (size_t)r->nk, ecount = (size_t)r->nk * r->nb;
-            st = grow_pinned(sl.hk, sl.hk_cap, kcount);
-            if (st == UEPM_OK) st = grow_pinned(sl.hev, sl.hev_cap, ecount);
-            if (st == UEPM_OK && (sl.dk.ensure(kcount * sizeof(double)) != hipSuccess ||
-                                  sl.dev.ensure(ecount * sizeof(double)) != hipSuccess))
+            const size_t kcap = std::max<size_t>(kcount, 3 * 4096), ecap = std::max<size_t>(ecount, 4096 * 32);
+            st = grow_pinned(sl.hk, sl.hk_cap, kcap);
+            if (st == UEPM_OK) st = grow_pinned(sl.hev, sl.hev_cap, ecap);
+            if (st == UEPM_OK && (sl.dk.ensure(kcap * sizeof(double)) != hipSuccess ||
+                                  sl.dev.ensure(ecap * sizeof(double)) != hipSuccess))
                 st = fail(UEPM_ERR_NO_MEMORY, "server: request buffers");
+            if (st == UEPM_OK && !sl.arena.p) {
+                if (sl.arena.ensure(slot_share(S)) != hipSuccess)
+                    st = fail(UEPM_ERR_NO_MEMORY, "server: the slot's workspace arena (" +
+                                                      std::to_string(slot_share(S)) + " bytes)");
+                else
+                    S->used += sl.arena.bytes;
+            }
             if (st == UEPM_OK) {
                 std::memcpy(sl.hk, r->k.data(), kcount * sizeof(double));
                 if (hipMemcpyAsync(sl.dk.p, sl.hk, kcount * sizeof(double), hipMemcpyHostToDevice, sl.stream) !=
@@ -414,8 +410,8 @@ void worker_main(uepm_server* S) {
                     st = fail(UEPM_ERR_DEVICE, "server: copy in");
             }
             if (st == UEPM_OK)
-                st = uepm_solve_device(pp->plan, sl.dk.as<double>(), r->nk, sl.dev.as<double>(), nullptr, nullptr,
-                                       sl.stream);
+                st = solve_region(pp->plan, sl.dk.as<double>(), r->nk, sl.dev.as<double>(), sl.stream,
+                                  sl.arena.as<char>(), sl.arena.bytes);
             if (st == UEPM_OK &&
                 (hipMemcpyAsync(sl.hev, sl.dev.p, ecount * sizeof(double), hipMemcpyDeviceToHost, sl.stream) !=
                      hipSuccess ||

@@ -381,6 +381,25 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
 
 }  // namespace uepm_host
 
+namespace uepm_host {
+int solve_region(uepm_plan* p, const double* dk, int64_t nk, double* devals, hipStream_t s, char* base,
+                 size_t bytes) {
+    const int v = solve_args_ok(p, nk, dk, devals, nullptr, nullptr, "server solve");
+    if (v != UEPM_OK || nk == 0) return v;
+    if (!base) return fail(UEPM_ERR_INVALID_ARG, "server solve: no workspace region");
+    const size_t per_k = plan_carve(p, 1, nullptr, nullptr);
+    if (per_k > bytes)
+        return fail(UEPM_ERR_NO_MEMORY, "server solve: one k-point needs " + std::to_string(per_k) +
+                                            " bytes of workspace, the region holds " + std::to_string(bytes));
+    HIP_OK(hipSetDevice(p->ctx->device));
+    int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(plan_chunk(p, nk), (int64_t)(bytes / per_k)));
+    while (chunk > 1 && plan_carve(p, chunk, nullptr, nullptr) > bytes) --chunk;
+    LaneView view;
+    plan_carve(p, chunk, base, &view);
+    return solve_chunks(p, dk, nk, devals, nullptr, nullptr, s, &view, 1, chunk, false);
+}
+}  // namespace uepm_host
+
 extern "C" {
 
 const char* uepm_last_error(void) { return last_error().c_str(); }

@@ -174,6 +174,11 @@ int solve_args_ok(const uepm_plan* p, int64_t nk, const double* dk, const double
 // device memory is created (stage timing is then not recorded).
 int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, double* devecs, double* dgrads,
                  hipStream_t s, const LaneView* views, int lanes, int64_t chunk, bool no_alloc);
+// Eigenvalues of nk k-points on stream s with the chunk workspace carved out of
+// the device region [base, base + bytes): one chunk in flight, chunks of the
+// plan's size or smaller so that the workspace fits.  Allocates no device memory.
+int solve_region(uepm_plan* p, const double* dk, int64_t nk, double* devals, hipStream_t s, char* base,
+                 size_t bytes);
 // Looks up the no-allocation scratch array's address on `device` (current),
 // once per device; called by uepm_ctx_create.
 int noalloc_setup(int device);
