@@ -196,6 +196,30 @@ def test_server_result_alone_equals_result_in_the_mix(gpu_ctx):
     srv.close()
 
 
+@pytest.mark.parametrize("flags", [api.NONLOCAL, api.NONLOCAL | api.MIXED])
+def test_server_chunks_within_slot_arena(gpu_ctx, flags):
+    """A slot's workspace arena (budget / slots) smaller than a request: the
+    request runs in chunks carved out of the arena, and every k-point comes out
+    bit-identical to a plain solve (each k-point's work does not depend on the
+    chunk it runs in).  Two slots, four requests of 1000 k-points."""
+    print(f"case: server arena chunking, Si n=12 flags {flags}", flush=True)
+    mat, nb = load("Si"), 12
+    G = api.basis_vectors(12)
+    p = api.Plan(gpu_ctx, mat, G, flags, nb)
+    per_k = p.workspace_bytes(1)
+    ks = [api.bz_sample(900 + i, 1000) for i in range(4)]
+    ref = [p.solve(k)[0] for k in ks]
+    p.close()
+    srv = api.Server(gpu_ctx, 2 * 96 * per_k, max_inflight=2)  # under 96 k-points per chunk
+    tickets = [srv.submit(mat, 12, flags, nb, k) for k in ks]
+    for t, r in zip(tickets, ref):
+        ev, _, _ = srv.wait(t)
+        assert np.array_equal(ev, r)
+    st = srv.stats()
+    assert st["completed"] == 4 and st["used_bytes"] <= 2 * 96 * per_k
+    srv.close()
+
+
 def test_server_mixed_vs_fp64_under_load(gpu_ctx):
     print("case: server mixed vs FP64, 40 requests at 400 req/s", flush=True)
     wl = _workload(12, 40)
