@@ -372,7 +372,6 @@ void worker_main(uepm_server* S) {
             key.bytes.append(reinterpret_cast<const char*>(&r->nb), sizeof(int));
             int st = UEPM_OK;
             PooledPlan* pp = acquire_plan(S, *r, key, &st);
-            if (!pp && st == UEPM_OK) break;  // memory: wait for completions
             {
                 std::lock_guard<std::mutex> g(S->mu);
                 S->queue.pop_front();
