@@ -149,6 +149,25 @@ def test_tuned_plan_above_the_register_grids(gpu_ctx):
     p.close()
 
 
+@pytest.mark.parametrize("sym,n,flags,tol", [("Ge", 11, api.NONLOCAL | api.SOC | api.MIXED, 1e-8),
+                                          ("Si", 14, api.NONLOCAL, 1e-10)])
+def test_blocked_reduction_workgroup_sizes(gpu_ctx, sym, n, flags, tol):
+    """The LDS-blocked reduction with 256- and 512-lane workgroups (its global
+    leading columns streamed, the LDS block split in items over 4 or 8 waves)
+    gives the eigenvalues of the automatic configuration: the precision form at
+    dim 338 with the blocked kernel alone for every step, the FP64 form at dim
+    259 for its leading steps."""
+    print(f"case: blocked reduction 256 / 512 lanes, {sym} n={n} flags {flags}", flush=True)
+    p = api.Plan(gpu_ctx, load(sym), api.basis_vectors(n), flags, 16)
+    ks = api.bz_sample(77, 128)
+    ev0, _, _ = p.solve(ks)
+    for tb in (256, 512):
+        p.set_config(tridiag=1, tri_block=tb)
+        ev, _, _ = p.solve(ks)
+        assert np.abs(ev - ev0).max() <= tol, (tb, np.abs(ev - ev0).max())
+    p.close()
+
+
 def test_batch_graph_matches_individual_solves(gpu_ctx):
     print("case: batch graph over 4 uneven plans, 2 replays", flush=True)
     specs = [("Si", 8, api.NONLOCAL | api.MIXED, 12), ("Ge", 8, api.NONLOCAL | api.SOC | api.MIXED, 16),
