@@ -111,7 +111,11 @@ int uepm_bz_sample(uint64_t seed, int64_t n, double* k_out);
  * reference applies in Initialize, BandStructure.cpp:78-82, i.e. nb counts
  * rows of the 2N x 2N spectrum when UEPM_SOC is set).
  * SOC with a period outside {2,3,4} -> UEPM_ERR_RUNTIME
- * (SpinOrbitFunctional.cpp:59,71). */
+ * (SpinOrbitFunctional.cpp:59,71).  A matrix dimension (N, or 2N with SOC)
+ * above UEPM_MAX_DIM -> UEPM_ERR_OUT_OF_RANGE: the reductions and
+ * back-transforms keep up to 9 rows per lane of a wave (n = 15 shells with SOC
+ * is 566, n = 16 is 614; the reference's Eigen solver has no such limit). */
+#define UEPM_MAX_DIM 576
 int uepm_plan_create(uepm_ctx* ctx, const uepm_material* mat, const int32_t* G, int N,
                      unsigned flags, int nb, uepm_plan** out);
 int uepm_plan_destroy(uepm_plan* plan);

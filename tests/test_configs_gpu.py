@@ -1,8 +1,9 @@
 """Parity at the large spinor dimensions and for the batching and serving
 workloads, against the oracle.
 
-* Dimensions 229 and 259 (Si, n = 13, 14) and 338, 362, 458, 518 (GaAs / Ge
-  with spin-orbit, n = 11..14): the assembled H(k) element by element, and
+* Dimensions 229 and 259 (Si, n = 13, 14) and 338, 362, 458, 518, 566 (GaAs /
+  Ge with spin-orbit, n = 11..15; 566 is the largest a plan takes): the
+  assembled H(k) element by element, and
   the FP64 and precision-form eigenvalues with the FP64 eigenvectors'
   residuals.  These are the bases the serving workload draws
   (BandStructure.cpp:26-57, SpinOrbitFunctional.cpp:20-138).
@@ -42,7 +43,8 @@ def _fp64_tol(oracle, mat, G, k, nl):
 
 
 @pytest.mark.parametrize("sym,n,soc", [("Si", 13, False), ("Si", 14, False), ("GaAs", 11, True),
-                                       ("Ge", 12, True), ("Ge", 13, True), ("GaAs", 14, True)])
+                                       ("Ge", 12, True), ("Ge", 13, True), ("GaAs", 14, True),
+                                       ("Ge", 15, True)])
 def test_large_dimensions_vs_oracle(gpu_ctx, oracle, sym, n, soc):
     mat = load(sym)
     G = api.basis_vectors(n)

@@ -37,6 +37,13 @@ def test_config_validation_and_workspace(gpu_ctx):
     r = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(11), api.NONLOCAL | api.SOC | api.MIXED, 16)  # dim 338
     with pytest.raises(UepmError):
         r.set_config(tridiag=2)
+    # dims to 576 are taken (n = 15 with SOC: 566); n = 16 with SOC (614) is refused at creation
+    big = api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(15), api.NONLOCAL | api.SOC | api.MIXED, 16)
+    assert big.dim == 566
+    big.close()
+    with pytest.raises(UepmError) as ei:
+        api.Plan(gpu_ctx, load("Ge"), api.basis_vectors(16), api.NONLOCAL | api.SOC, 16)
+    assert ei.value.code == 3 and "576" in str(ei.value)  # UEPM_ERR_OUT_OF_RANGE
     p.close()
     q.close()
     r.close()

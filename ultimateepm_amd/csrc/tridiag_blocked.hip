@@ -27,6 +27,7 @@
 #include <cstdint>
 
 #include "solver.hpp"
+#include "uepm.h"
 #include "wave_ops.hpp"
 
 namespace uepm {
@@ -94,6 +95,7 @@ __host__ __device__ inline int padded_len(int L, int MOD) { return L + (((2 - L)
 constexpr int kTriNBP = 4;
 constexpr int kTriParts = 4;
 constexpr int kTriMaxRowsPerLane = 9;  // rows per lane in the symmetric product: n <= 576
+static_assert(64 * kTriMaxRowsPerLane >= UEPM_MAX_DIM, "every plan dimension fits the symmetric product");
 
 template <typename T>
 struct TriSmem {

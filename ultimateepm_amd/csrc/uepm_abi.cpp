@@ -578,6 +578,10 @@ int uepm_plan_create(uepm_ctx* ctx, const uepm_material* m, const int32_t* G, in
     const bool soc = flags & UEPM_SOC;
     const int dim = soc ? 2 * N : N;
     if (nb < 1 || nb > dim) return fail(UEPM_ERR_OUT_OF_RANGE, "uepm_plan_create: nb must be in [1, dim]");
+    if (dim > UEPM_MAX_DIM)
+        return fail(UEPM_ERR_OUT_OF_RANGE, "uepm_plan_create: matrix dimension " + std::to_string(dim) +
+                                               " exceeds " + std::to_string(UEPM_MAX_DIM) +
+                                               ", the largest the reductions take");
     if (!(m->lattice_constant_angstrom > 0.0)) return fail(UEPM_ERR_INVALID_ARG, "lattice constant must be positive");
     if (soc) {
         auto ok = [](int p) { return p == 2 || p == 3 || p == 4; };
