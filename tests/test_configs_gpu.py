@@ -2,8 +2,10 @@
 workloads, against the oracle.
 
 * Dimensions 229 and 259 (Si, n = 13, 14) and 338, 362, 458, 518, 566 (GaAs /
-  Ge with spin-orbit, n = 11..15; 566 is the largest a plan takes): the
-  assembled H(k) element by element, and
+  Ge with spin-orbit, n = 11..15; 566 is the largest a plan takes) and 411
+  (Si, n = 20: its 411 x 184 B of per-vector assembly state pass 64 KB of
+  LDS, the dynamic-LDS path of the assembly): the assembled H(k) element by
+  element, and
   the FP64 and precision-form eigenvalues with the FP64 eigenvectors'
   residuals.  These are the bases the serving workload draws
   (BandStructure.cpp:26-57, SpinOrbitFunctional.cpp:20-138).
@@ -44,7 +46,7 @@ def _fp64_tol(oracle, mat, G, k, nl):
 
 @pytest.mark.parametrize("sym,n,soc", [("Si", 13, False), ("Si", 14, False), ("GaAs", 11, True),
                                        ("Ge", 12, True), ("Ge", 13, True), ("GaAs", 14, True),
-                                       ("Ge", 15, True)])
+                                       ("Ge", 15, True), ("Si", 20, False)])
 def test_large_dimensions_vs_oracle(gpu_ctx, oracle, sym, n, soc):
     mat = load(sym)
     G = api.basis_vectors(n)
