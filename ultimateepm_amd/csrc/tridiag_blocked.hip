@@ -9,8 +9,9 @@
 // reading A(c, r) at a fixed c (column walk; the start offset of column r
 // advances by 2 per column, the in-column offset c - r falls by 1).
 // The trailing columns [c0, n) live in LDS; the leading columns [0, c0) that
-// do not fit live in a per-workgroup global scratch region (L2-resident while
-// the workgroup runs) and are only touched during the first c0 steps.
+// do not fit live in a per-workgroup global scratch region (hundreds of KB at
+// the large dims, beyond a workgroup's share of the L2, so each step streams
+// its live part from memory) and are touched only during the first c0 steps.
 //
 // Algorithm (LAPACK xHETRD/xLATRD lower, restated).  Columns are reduced in
 // panels of NBP.  Inside a panel the trailing matrix is read-only: step j
