@@ -223,9 +223,10 @@ __device__ __forceinline__ double lane_rand(uint32_t i, uint32_t b) {
     return (double)(h >> 8) * (1.0 / 8388608.0) - 1.0;
 }
 
-// Wave-per-k variant: the d, e of a k-point are wave-uniform (scalar loads,
-// SGPR operands); lanes = bands x S shifts (multisection: one sweep narrows the
-// bracket of every band S+1-fold); inverse iteration on the first lane of a band.
+// Wave-per-k variant: the d, e of a k-point are staged in LDS (16 n bytes of
+// dynamic LDS) and read from there by every Sturm count and inverse-iteration
+// sweep; lanes = bands x S shifts (multisection: one sweep narrows the bracket
+// of every band S+1-fold); inverse iteration on the first lane of a band.
 __device__ __forceinline__ double rcp_nr64(double x) {
     double r = __builtin_amdgcn_rcp(x);
     r = fma(fma(-x, r, 1.0), r, r);
@@ -317,6 +318,17 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
         }
         return;
     }
+    // d and e in LDS: the Sturm counts re-read them every probe sweep, and the
+    // inverse iteration every sweep (3-8% off a launch alone, against reading
+    // them from global memory; the same values, so the same results)
+    extern __shared__ double tri_lds[];
+    double* const sd = tri_lds;
+    double* const se = tri_lds + n;
+    for (int i = lane; i < n; i += 64) {
+        sd[i] = d[i];
+        se[i] = e[i];
+    }
+    __syncthreads();
     const double pivmin = DBL_MIN * fmax(1.0, emax2);
     const double eps = DBL_EPSILON;
     lo -= 2.0 * eps * tnorm + pivmin;
@@ -331,7 +343,7 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
         double L = lo, U = hi;
         for (int sw = 0; sw < 2; ++sw) {
             const double x = L + (double)(lane + 1) * ((U - L) / 65.0);
-            const int cnt = sturm_count_u(d, e, n, x, pivmin);
+            const int cnt = sturm_count_u(sd, se, n, x, pivmin);
             double nlo = lo, nhi = hi;
             for (int u = 0; u < 64; ++u) {
                 const int cu = __shfl(cnt, u, 64);
@@ -351,7 +363,7 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
         const bool done = !live || w <= 2.0 * eps * fmax(fabs(lo), fabs(hi)) + 4.0 * pivmin || w <= abstol;
         if (__all(done)) break;
         const double x = lo + (double)(sub + 1) * (w / (double)(S + 1));
-        const int cnt = done ? 0 : sturm_count_u(d, e, n, x, pivmin);
+        const int cnt = done ? 0 : sturm_count_u(sd, se, n, x, pivmin);
         // gather the S probes of this band
         double nlo = lo, nhi = hi;
 #pragma unroll
@@ -397,8 +409,8 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
     for (int iter = 0; iter < inv_sweeps; ++iter) {
         const bool first = iter == 0;
         // forward: factor row k, apply the interchange and the multiplier
-        double a = d[0] - lam;
-        double c = n > 1 ? e[0] : 0.0;
+        double a = sd[0] - lam;
+        double c = n > 1 ? se[0] : 0.0;
         double yk = first ? lane_rand(0u, (uint32_t)b) : X[IDX(0)] * sc;
         for (int k0 = 0; k0 + 1 < n; k0 += PB) {
             double xn[PB];
@@ -410,9 +422,9 @@ __global__ __launch_bounds__(64, 4) void tri_eig_wave_kernel(const double* __res
             for (int u = 0; u < PB; ++u) {
                 const int k = k0 + u;
                 if (k + 1 >= n) break;
-                const double sub_ = e[k];
-                const double dn = d[k + 1] - lam;
-                const double en = (k + 2 < n) ? e[k + 1] : 0.0;
+                const double sub_ = se[k];
+                const double dn = sd[k + 1] - lam;
+                const double en = (k + 2 < n) ? se[k + 1] : 0.0;
                 double yk1 = first ? lane_rand((uint32_t)(k + 1), (uint32_t)b) : xn[u] * sc;
                 double mult;
                 if (fabs(a) >= fabs(sub_)) {
@@ -524,14 +536,15 @@ hipError_t launch_assemble(const DevParams& P, const int32_t* dG, const double* 
 hipError_t launch_tri_eig(const double* dd, const double* de, int n, int nb, int64_t nk, double* devals, double* dZ,
                           double* dW, int inv_sweeps, double abstol_rel, int* kstat, unsigned long long* status,
                           int64_t k0, hipStream_t s) {
+    const size_t shm = sizeof(double) * 2 * (size_t)n;  // d and e of the k-point
     if (nb <= 16)
-        hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
+        hipLaunchKernelGGL(tri_eig_wave_kernel<4>, dim3((unsigned)nk, 1), dim3(64), shm, s, dd, de, n, nb, nk, devals,
                            dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     else if (nb <= 32)
-        hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), 0, s, dd, de, n, nb, nk, devals,
+        hipLaunchKernelGGL(tri_eig_wave_kernel<2>, dim3((unsigned)nk, 1), dim3(64), shm, s, dd, de, n, nb, nk, devals,
                            dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     else
-        hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), 0, s, dd,
+        hipLaunchKernelGGL(tri_eig_wave_kernel<1>, dim3((unsigned)nk, (unsigned)((nb + 63) / 64)), dim3(64), shm, s, dd,
                            de, n, nb, nk, devals, dZ, dW, inv_sweeps, abstol_rel, kstat, status, k0);
     return hipGetLastError();
 }
