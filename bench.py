@@ -31,6 +31,10 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 DATAPATH = {
     ("mixed", "tridiag_reg_kernel"): "VALU FP32 (157.3 TF/s, the FP32 MFMA rate too): no MFMA in the register-grid chain",
     ("fp64", "tridiag_reg_kernel"): "VALU FP64 (78.6 TF/s, equal to the FP64 MFMA rate)",
+    ("mixed", "tridiag_blocked_kernel"): "VALU FP32 symmetric products, MFMA FP32 panel updates (157.3 TF/s either way)",
+    ("mixed", "tridiag_blocked+reg_kernel"): "VALU FP32 symmetric products, MFMA FP32 panel updates, then the register grids",
+    ("fp64", "tridiag_blocked_kernel"): "VALU FP64 symmetric products, MFMA FP64 panel updates (78.6 TF/s either way)",
+    ("fp64", "tridiag_blocked+reg_kernel"): "VALU FP64 symmetric products, MFMA FP64 panel updates, then the register grids",
 }
 HBM_PEAK_GBS = 8000.0
 SEED = 20261202
@@ -193,7 +197,7 @@ def serving_leg(ctx, requests: int, seed: int = SEED, budget_gib: int = 64, infl
     return out
 
 
-def config3_leg(ctx, nk: int = 100_000, seed: int = SEED) -> dict:
+def ge_soc_leg(ctx, nk: int = 100_000, seed: int = SEED) -> dict:
     """Precision form at the spinor size: Ge with spin-orbit (materials.yaml,
     non-local on), n = 10 shells (dim 2 x 137 = 274), nk seeded first-BZ
     k-points, lowest 16 eigenvalues; both forms, one untimed and one timed
@@ -580,7 +584,7 @@ def main() -> int:
     ap.add_argument("--shells", type=int, default=12)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--timing-steps", type=int, default=2, help="extra steps with per-stage HIP-event timing")
-    ap.add_argument("--config3-nk", type=int, default=100_000,
+    ap.add_argument("--ge-soc-nk", type=int, default=100_000,
                     help="k-points of the Ge + SOC (dim 274) leg of the default run (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--modes", default="fp64,mixed", help="comma list of fp64, mixed; the value is mixed's")
@@ -739,7 +743,7 @@ def main() -> int:
         "avg_launch_ms": round(dom_ms / max(dom_launches, 1), 4),
         "k_per_launch": round(args.nk * ts / max(dom_launches, 1), 1),
         "flop_per_k": tridiag_flops(r["dim"]),
-        "datapath": DATAPATH.get((head, dominant.split("<")[0]), "see DESIGN.md"),
+        "datapath": DATAPATH.get((head, dominant.split("<")[0]), "not listed for this kernel"),
         "stage_ms_per_step": {k: round(v[0] / ts, 3) for k, v in stage_ms.items()},
         "launches_in_flight": r["lanes"],
         "chip_reduction_TFLOPs_over_step": round(chip, 3),
@@ -786,9 +790,9 @@ def main() -> int:
         },
         "roofline": roofline,
     }
-    if args.config3_nk > 0:
+    if args.ge_soc_nk > 0:
         log("Ge + SOC (dim 274) leg")
-        out["config3_ge_soc"] = config3_leg(ctx, args.config3_nk)
+        out["ge_soc_n10"] = ge_soc_leg(ctx, args.ge_soc_nk)
     if args.batch_leg_nk > 0:
         log("batching leg (28 plans, tuned, one graph launch per form)")
         out["batching"] = batching_leg(ctx, args.batch_leg_nk)
