@@ -207,11 +207,12 @@ typedef struct uepm_config {
                           reduction only; 2 register-resident complex64 reduction
                           (UEPM_MIXED plans, dim <= 288)                              */
     int32_t tri_block; /* LDS-blocked reduction workgroup size: 256 or 512 (0 = 512)  */
-    int32_t bt_bands;  /* back-transform: 0 automatic (UEPM_MIXED plans above dim 192
-                          or with nb + 4 > 16: the blocked MFMA kernel while
-                          nb + 4 <= 32 and X fits the LDS; otherwise 16); 4, 8 or
-                          16 bands per workgroup of the reflector-by-reflector
-                          kernel                                                      */
+    int32_t bt_bands;  /* back-transform: 0 automatic (above dim 192, and UEPM_MIXED
+                          plans with nb + 4 > 16: the blocked MFMA kernel while X
+                          fits the LDS -- complex64 while nb + 4 <= 32, FP64 in
+                          passes of 16 bands up to dim 493 at 160 KB; otherwise
+                          16); 4, 8 or 16 bands per workgroup of the
+                          reflector-by-reflector kernel                               */
     int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */
     int32_t lanes;     /* chunks in flight: 0 automatic (2 when a solve spans two or
                           more chunks: alternate chunks on the caller's stream and on

@@ -136,6 +136,32 @@ def test_wy_back_transform_matches_per_reflector(gpu_ctx, sym, n, soc, nb):
     p.close()
 
 
+@pytest.mark.parametrize("sym,n,soc,nb", [("Ge", 10, True, 16), ("Si", 20, False, 24), ("Ge", 13, True, 12),
+                                          ("Ge", 14, True, 16)])
+def test_fp64_wy_back_transform_matches_per_reflector(gpu_ctx, sym, n, soc, nb):
+    """FP64 form above dim 192: the compact-WY back-transform on FP64 MFMA
+    (automatic, bt_bands 0; dims 274, 411 in two passes of 16 bands, 458) and the
+    reflector-by-reflector kernel (bt_bands 16) apply the same Q to the same
+    tridiagonal eigenvectors, so the eigenvectors agree elementwise to rounding
+    and the Hellmann-Feynman gradients with them; the eigenvalues come before
+    the back-transform and are identical.  At dim 518 X does not fit the LDS and
+    the automatic choice is the reflector kernel itself."""
+    print(f"case: FP64 WY vs per-reflector back-transform, {sym} n={n} soc={soc} nb={nb}", flush=True)
+    flags = api.VECTORS | api.GRADIENTS | (api.SOC if soc else 0)
+    p = api.Plan(gpu_ctx, load(sym), api.basis_vectors(n), flags, nb)
+    ks = api.bz_sample(909, 48)
+    ev0, v0, g0 = p.solve(ks)  # automatic
+    p.set_config(bt_bands=16)
+    ev1, v1, g1 = p.solve(ks)
+    assert np.array_equal(ev0, ev1)
+    assert np.isfinite(v0).all() and np.isfinite(g0).all()
+    assert np.abs(v0 - v1).max() <= 1e-11
+    assert np.abs(g0 - g1).max() <= 1e-9 * max(1.0, np.abs(g1).max())
+    # unit eigenvectors (the reflectors are unitary)
+    assert np.abs(np.linalg.norm(v0, axis=2) - 1.0).max() <= 1e-12
+    p.close()
+
+
 def test_tuned_plan_above_the_register_grids(gpu_ctx):
     """Above dim 288 the tuner chooses between the LDS-blocked reduction alone
     and its leading steps followed by the register grids; every candidate

@@ -459,6 +459,191 @@ __global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __rest
     }
 }
 
+// The FP64 form's blocked back-transform on v_mfma_f64_16x16x4f64: the three
+// steps of apply_q_wy_kernel in complex128, sixteen bands per pass (bands
+// b0 .. b0 + nbp - 1 of the nb per k-point), X in LDS.  The FP64 instruction
+// puts output item v of lane l at element (4 v + l / 16, l % 16) (the FP32
+// one at (4 (l / 16) + v, l % 16)).  The four waves' partial [G | W] are added
+// in two levels, (w0 + w2) + (w1 + w3), a fixed order in half the LDS of a
+// four-way table.  As the FP64 form's last stage it also counts the k-points
+// that kstat flags into status (first pass only) and writes the gradients
+// (the reflector kernel's sums, in its order).
+__global__ __launch_bounds__(256) void apply_q_wy_f64_kernel(
+    const double2* __restrict__ V, int n, int ld, const double2* __restrict__ tauall, const double* __restrict__ Z,
+    int nb, int b0, int nbp, double2* __restrict__ X, double* __restrict__ grads, DevParams P,
+    const double* __restrict__ kpts, const int32_t* __restrict__ G, const int* __restrict__ kstat,
+    unsigned long long* __restrict__ status, int64_t k0) {
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    constexpr int XS = 17;  // padded row stride of X in LDS (complex elements)
+    extern __shared__ __align__(16) unsigned char wy64_smem[];
+    double2* Xs = reinterpret_cast<double2*>(wy64_smem);  // X[row][band] at row * XS + band
+    __shared__ d4 PP[2][4][64];  // partials (G re, G im, W re, W im): waves 2, 3, then w0 + w2, w1 + w3
+    __shared__ double2 Gs[16][17];  // G = V^H V
+    __shared__ double2 Ws[16][XS];  // W = V^H X
+    __shared__ double2 Wq[16][XS];  // W' = T W
+    __shared__ double2 tb[16];      // tau of the block's reflectors
+
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int l16 = lane & 15, l4 = lane >> 4;
+    const int64_t kk = blockIdx.x;
+    if (status && b0 == 0 && tid == 0 && kstat[kk]) {
+        atomicAdd(&status[0], 1ull);
+        atomicMin(&status[1], (unsigned long long)(k0 + kk));
+    }
+    const double2* Vk = V + kk * (int64_t)ld * n;
+    const double2* tau = tauall + kk * n;
+    auto mfma = [](double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); };
+    // element `row` of v_j: 1 at row j + 1, the stored reflector below, zero above
+    // and for j past the last reflector (n - 2)
+    auto vload = [&](int row, int j) -> double2 {
+        if (j > n - 2 || row >= n || row <= j) return make_double2(0.0, 0.0);
+        if (row == j + 1) return make_double2(1.0, 0.0);
+        return Vk[row + (int64_t)j * ld];
+    };
+
+    for (int e = tid; e < n * 16; e += 256) {
+        const int b = e / n, row = e - b * n;
+        const double z = b < nbp ? Z[(kk * nb + b0 + b) * (int64_t)n + row] : 0.0;
+        Xs[row * XS + b] = make_double2(z, 0.0);
+    }
+    __syncthreads();
+
+    const int nblk = (n - 1 + 15) >> 4;  // reflectors 0 .. n - 2
+    for (int blk = nblk - 1; blk >= 0; --blk) {
+        const int j0 = 16 * blk, r0 = j0 + 1;
+        // ---- 1. partial [G | W] over this wave's 4-row chunks ----
+        const d4 z4 = {0.0, 0.0, 0.0, 0.0};
+        d4 gre = z4, gim = z4, wre = z4, wim = z4;
+        const int nch = (n - r0 + 3) >> 2;
+        for (int ch0 = wv; ch0 < nch; ch0 += 16) {
+            double2 vv[4], xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int row = r0 + 4 * (ch0 + 4 * u) + l4;
+                vv[u] = vload(row, j0 + l16);
+                xx[u] = row < n ? Xs[row * XS + l16] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double2 v = vv[u], x = xx[u];
+                gre = mfma(v.x, v.x, gre);
+                gre = mfma(v.y, v.y, gre);
+                gim = mfma(v.x, v.y, gim);
+                gim = mfma(-v.y, v.x, gim);
+                wre = mfma(v.x, x.x, wre);
+                wre = mfma(v.y, x.y, wre);
+                wim = mfma(v.x, x.y, wim);
+                wim = mfma(-v.y, x.x, wim);
+            }
+        }
+        if (wv >= 2) {
+            PP[wv - 2][0][lane] = gre;
+            PP[wv - 2][1][lane] = gim;
+            PP[wv - 2][2][lane] = wre;
+            PP[wv - 2][3][lane] = wim;
+        }
+        if (tid < 16) tb[tid] = j0 + tid <= n - 2 ? tau[j0 + tid] : make_double2(0.0, 0.0);
+        __syncthreads();
+        if (wv < 2) {  // each lane reads, then overwrites, its own slots
+            PP[wv][0][lane] = gre + PP[wv][0][lane];
+            PP[wv][1][lane] = gim + PP[wv][1][lane];
+            PP[wv][2][lane] = wre + PP[wv][2][lane];
+            PP[wv][3][lane] = wim + PP[wv][3][lane];
+        }
+        __syncthreads();
+        {
+            const int f = tid >> 6, L = tid & 63, c = L & 15;
+            const d4 sum = PP[0][f][L] + PP[1][f][L];
+            double2* dst = f < 2 ? &Gs[0][0] : &Ws[0][0];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = 4 * q + (L >> 4);
+                reinterpret_cast<double*>(&dst[m * 17 + c])[f & 1] = sum[q];
+            }
+        }
+        __syncthreads();
+        // ---- 2. W' = T W: back substitution U W' = W, one band per lane ----
+        if (tid < 16) {
+            const int b = tid;
+            double2 wq[16];
+#pragma unroll
+            for (int i = 15; i >= 0; --i) {
+                double sr = Ws[i][b].x, si = Ws[i][b].y;
+#pragma unroll
+                for (int p = i + 1; p < 16; ++p) {
+                    const double2 u = Gs[i][p];
+                    sr = fma(-u.x, wq[p].x, fma(u.y, wq[p].y, sr));
+                    si = fma(-u.x, wq[p].y, fma(-u.y, wq[p].x, si));
+                }
+                const double2 t = tb[i];  // 1 / U_ii
+                wq[i] = make_double2(t.x * sr - t.y * si, t.x * si + t.y * sr);
+                Wq[i][b] = wq[i];
+            }
+        }
+        __syncthreads();
+        // ---- 3. X -= V W' on the rows below j0, 16-row tiles over the waves ----
+        const int ntile = (n - r0 + 15) >> 4;
+        for (int tt = wv; tt < ntile; tt += 4) {
+            const int R0 = r0 + 16 * tt;
+            double2 a[4];  // A[m][k] = v_{j0+k}[R0 + m]: lane (m = l16, k = 4 kc + l4)
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) a[kc] = vload(R0 + l16, j0 + 4 * kc + l4);
+            d4 dre, dim;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = R0 + 4 * q + l4;
+                const double2 x = row < n ? Xs[row * XS + l16] : make_double2(0.0, 0.0);
+                dre[q] = x.x;
+                dim[q] = x.y;
+            }
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc) {
+                const double2 b = Wq[4 * kc + l4][l16];
+                dre = mfma(-a[kc].x, b.x, dre);
+                dre = mfma(a[kc].y, b.y, dre);
+                dim = mfma(-a[kc].x, b.y, dim);
+                dim = mfma(-a[kc].y, b.x, dim);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = R0 + 4 * q + l4;
+                if (row < n) Xs[row * XS + l16] = make_double2(dre[q], dim[q]);
+            }
+        }
+        __syncthreads();
+    }
+    // eigenvectors X[k][band][row]
+    for (int e = tid; e < n * nbp; e += 256) {
+        const int b = e / n, row = e - b * n;
+        X[(kk * nb + b0 + b) * (int64_t)n + row] = Xs[row * XS + b];
+    }
+    if (grads) {
+        // Hellmann-Feynman on the kinetic term, 16 lanes per band (rows rs + 16 q
+        // summed in ascending q, then across the 16 lanes); spinor rows i >= N belong to G_{i-N}
+        const int b = tid >> 4, rs = tid & 15;
+        const double* k = kpts + 3 * kk;
+        const int N = P.N;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int i = rs; i < n; i += 16) {
+            const double2 x = Xs[i * XS + b];
+            const int g = i < N ? i : i - N;
+            const double w = x.x * x.x + x.y * x.y;
+            sx += (k[0] + (double)G[3 * g]) * w;
+            sy += (k[1] + (double)G[3 * g + 1]) * w;
+            sz += (k[2] + (double)G[3 * g + 2]) * w;
+        }
+        sx = row_sum16(sx);
+        sy = row_sum16(sy);
+        sz = row_sum16(sz);
+        if (rs == 0 && b < nbp) {
+            double* gg = grads + 3 * (kk * nb + b0 + b);
+            gg[0] = sx * P.grad_pref * P.frac_to_phys;
+            gg[1] = sy * P.grad_pref * P.frac_to_phys;
+            gg[2] = sz * P.grad_pref * P.frac_to_phys;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ launchers
 namespace {
 // Bands per 16-lane row of apply_q_kernel (1, 2 or 4): as many as keep the
@@ -512,10 +697,32 @@ hipError_t launch_cluster_orth(const double* dd, const double* de, const double*
     return hipGetLastError();
 }
 
+namespace {
+// dynamic LDS of apply_q_wy_f64_kernel (X in LDS) and its static part
+size_t wy64_lds_bytes(int n) { return sizeof(double2) * (size_t)n * 17; }
+constexpr size_t kWy64StaticBytes = sizeof(double) * 4 * 2 * 4 * 64 + sizeof(double2) * (3 * 16 * 17 + 16);
+}  // namespace
+
+bool apply_q_wy_f64_fits(int n, int lds_limit) {
+    return n >= 2 && wy64_lds_bytes(n) + kWy64StaticBytes <= (size_t)lds_limit;
+}
+
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                               int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
-                              int bands_per_group, hipStream_t s) {
+                              int bands_per_group, int lds_limit, hipStream_t s) {
+    if (bands_per_group == 0) {
+        // the blocked MFMA back-transform, sixteen bands per pass
+        if (!apply_q_wy_f64_fits(n, lds_limit)) return hipErrorInvalidValue;
+        const size_t shm = wy64_lds_bytes(n);
+        hipError_t err = hipFuncSetAttribute((const void*)apply_q_wy_f64_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (err != hipSuccess) return err;
+        for (int b0 = 0; b0 < nb; b0 += 16)
+            hipLaunchKernelGGL(apply_q_wy_f64_kernel, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb,
+                               b0, std::min(16, nb - b0), dX, dgrads, P, dk, dG, kstat, status, k0);
+        return hipGetLastError();
+    }
     return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, kstat, status, k0,
                                          bands_per_group, s);
 }

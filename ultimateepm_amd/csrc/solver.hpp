@@ -96,13 +96,16 @@ hipError_t launch_tridiag_reg_c128(const double2* H, int64_t hk, int m, int ld, 
 // reduction (FP64: over H; precision form: complex64 Vout), gradients when dgrads.
 hipError_t launch_cluster_orth(const double* dd, const double* de, const double* devals, double* dZ, int n, int nb,
                                int64_t nk, hipStream_t s);
-// bands_per_group: bands per workgroup (16 lanes each), 4, 8 or 16
+// bands_per_group: bands per workgroup (16 lanes each), 4, 8 or 16, or 0 for the
+// blocked MFMA back-transform (compact WY on FP64 MFMA, sixteen bands per pass),
+// which needs apply_q_wy_f64_fits under the per-workgroup LDS limit lds_limit.
 // The FP64 form's back-transform is its last stage: it adds the k-points kstat
 // flags to status (both may be null).
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
                               int64_t nk, const int* kstat, unsigned long long* status, int64_t k0,
-                              int bands_per_group, hipStream_t s);
+                              int bands_per_group, int lds_limit, hipStream_t s);
+bool apply_q_wy_f64_fits(int n, int lds_limit);
 // Precision form: bands_per_group 0 selects the blocked MFMA back-transform
 // (compact WY, sixteen reflectors per block), which needs apply_q_wy_fits
 // under the per-workgroup LDS limit lds_limit (bytes).

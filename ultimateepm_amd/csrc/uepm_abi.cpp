@@ -176,11 +176,13 @@ void plan_apply_config(uepm_plan* p) {
     p->tp32 = uepm::tri_plan(p->dim, (int)sizeof(float2), p->ctx->lds_limit, bs);
     p->bt_bands = p->cfg.bt_bands > 0 ? p->cfg.bt_bands : 16;
     const int nbr = mixed ? std::min(p->dim, p->nb + kMixedExtra) : p->nb;
-    // automatic: the blocked MFMA back-transform above dim 192 or past 16 Ritz vectors,
-    // where it is 2.5-3x faster; below, the reflector-by-reflector kernel (no LDS)
-    // shares the chip better with the other lane's register grids
-    p->bt_wy = mixed && p->cfg.bt_bands == 0 && (p->dim > 192 || nbr > 16) &&
-               uepm::apply_q_wy_fits(p->dim, nbr, p->ctx->lds_limit);
+    // automatic: the blocked MFMA back-transform above dim 192 (both forms) or, in the
+    // precision form, past 16 Ritz vectors, where it is 2.5-3.6x faster; below, the
+    // reflector-by-reflector kernel (no LDS) shares the chip better with the other
+    // lane's register grids, and in FP64 it is as fast or faster
+    const int lds = p->ctx->lds_limit;
+    p->bt_wy = p->cfg.bt_bands == 0 && (mixed ? (p->dim > 192 || nbr > 16) && uepm::apply_q_wy_fits(p->dim, nbr, lds)
+                                              : p->dim > 192 && uepm::apply_q_wy_f64_fits(p->dim, lds));
 }
 
 }  // namespace uepm_host
@@ -349,7 +351,8 @@ int solve_chunks(uepm_plan* p, const double* dk, int64_t nk, double* devals, dou
                                                  w.Z, n, nb, c, cs));
                 HIP_OK(uepm::launch_apply_q_f64(Hc, n, n, w.tau, w.Z, nb, Xc,
                                                 dgrads ? dgrads + 3 * c0 * nb : nullptr, p->P, kc, p->G.as<int32_t>(),
-                                                c, ks, status, c0, p->bt_bands, cs));
+                                                c, ks, status, c0, p->bt_wy ? 0 : p->bt_bands,
+                                                p->ctx->lds_limit, cs));
                 HIP_OK(rec(ci, kBack, 1));
             }
         }
@@ -747,10 +750,11 @@ int uepm_plan_tune(uepm_plan* p, const double* dk, int64_t nk, void* stream_, ue
     std::vector<uepm_config> cand;
     const bool mixed = p->flags & UEPM_MIXED;
     const bool reg_ok = mixed && p->dim <= uepm::tridiag_reg_max_n();
-    // back-transform: the blocked MFMA kernel (0, precision form) or 8 / 16 bands per
+    // back-transform: the blocked MFMA kernel (0: automatic, which is the blocked
+    // kernel in the precision form and above dim 192 in FP64) or 8 / 16 bands per
     // workgroup of the reflector-by-reflector one
     std::vector<int> bts = {8, 16};
-    if (mixed) bts.insert(bts.begin(), 0);
+    if (mixed || (p->dim > 192 && uepm::apply_q_wy_f64_fits(p->dim, p->ctx->lds_limit))) bts.insert(bts.begin(), 0);
     for (int bt : bts) {
         if (reg_ok) cand.push_back({2, 0, bt, p->cfg.max_chunk, p->cfg.lanes});
         for (int tb : {256, 512}) {

@@ -104,7 +104,7 @@ struct uepm_plan {
     uepm_config cfg{};    // requested configuration (0 fields = automatic)
     bool tri_reg = false; // register-resident complex64 reduction (precision form, dim <= 288)
     int bt_bands = 16;    // bands per workgroup of the reflector-by-reflector back-transform
-    bool bt_wy = false;   // precision form: the blocked MFMA back-transform instead
+    bool bt_wy = false;   // the blocked MFMA back-transform instead (complex64 or FP64 by form)
     int timing = 0;
     std::vector<hipEvent_t> ev;  // pairs per stage launch
     const char* stage_name[uepm_host::kNumStages] = {"assemble_kernel", "", "", "", ""};  // kernels of the last solve
