@@ -239,12 +239,13 @@ def ge_soc_leg(ctx, nk: int = 100_000, seed: int = SEED) -> dict:
     return out
 
 
-def dims_leg(ctx, seed: int = SEED, only=None) -> list:
+def dims_leg(ctx, seed: int = SEED, only=None, vectors: bool = False) -> list:
     """Throughput by basis dimension: Si (n = 6..14 shells, dims 59..259) and Ge with
-    spin-orbit (n = 6..14, dims 118..518), non-local, 16 bands, eigenvalues only,
-    2048 seeded BZ k-points (512 above dim 200), both forms; one untimed solve,
-    then one timed solve (wall clock over a device-resident batch) and the stage
-    split of a third.  `only`: the dimensions to run (default: all)."""
+    spin-orbit (n = 6..14, dims 118..518), non-local, 16 bands, eigenvalues only
+    (`vectors`: and eigenvectors), 2048 seeded BZ k-points (512 above dim 200),
+    both forms; one untimed solve, then one timed solve (wall clock over a
+    device-resident batch) and the stage split of a third.  `only`: the
+    dimensions to run (default: all)."""
     from ultimateepm_amd import api
     from ultimateepm_amd.material import load
 
@@ -256,27 +257,33 @@ def dims_leg(ctx, seed: int = SEED, only=None) -> list:
                 continue
             for mode in ("fp64", "mixed"):
                 flags = api.NONLOCAL | (api.SOC if soc else 0) | (api.MIXED if mode == "mixed" else 0)
+                if vectors:
+                    flags |= api.VECTORS
                 nb = 16
                 p = api.Plan(ctx, load(sym), G, flags, nb)
                 nk = 2048 if p.dim <= 200 else 512
                 dk = api.DeviceArray.from_host(ctx, api.bz_sample(seed + 5, nk))
                 de = api.DeviceArray(ctx, (nk, nb))
-                p.solve_device(dk.ptr, nk, de.ptr)
+                dv = api.DeviceArray(ctx, (nk, nb, p.dim, 2)) if vectors else None
+                vptr = dv.ptr if vectors else 0
+                p.solve_device(dk.ptr, nk, de.ptr, vptr)
                 ctx.synchronize()
                 t0 = time.perf_counter()
-                p.solve_device(dk.ptr, nk, de.ptr)
+                p.solve_device(dk.ptr, nk, de.ptr, vptr)
                 ctx.synchronize()
                 dt = time.perf_counter() - t0
                 p.set_timing(True)
-                p.solve_device(dk.ptr, nk, de.ptr)
+                p.solve_device(dk.ptr, nk, de.ptr, vptr)
                 ctx.synchronize()
                 rows.append({"material": sym, "shells": n, "soc": soc, "dim": p.dim, "mode": mode, "nk": nk,
-                             "kpts_per_s": round(nk / dt, 1),
+                             "vectors": vectors, "kpts_per_s": round(nk / dt, 1),
                              "stage_ms": {k: round(v[0], 3) for k, v in p.last_timing().items()}})
                 print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
                 p.close()
                 dk.free()
                 de.free()
+                if vectors:
+                    dv.free()
     return rows
 
 
@@ -595,6 +602,7 @@ def main() -> int:
                          "alone; dims: throughput by basis dimension")
     ap.add_argument("--requests", type=int, default=200, help="requests of --form serving")
     ap.add_argument("--dims", default="", help="comma list of basis dimensions for --form dims (default: all)")
+    ap.add_argument("--dims-vectors", action="store_true", help="--form dims: eigenvectors too")
     ap.add_argument("--batch-nk", type=int, default=20000, help="k-points per plan of --form batching")
     ap.add_argument("--batch-leg-nk", type=int, default=20000,
                     help="k-points per plan of the batching leg of the default run (0: skip)")
@@ -625,9 +633,10 @@ def main() -> int:
     if args.form == "dims":
         ctx = api.Context(0)
         only = {int(x) for x in args.dims.split(",") if x.strip()}
-        rows = dims_leg(ctx, only=only)
+        rows = dims_leg(ctx, only=only, vectors=args.dims_vectors)
         head = next((r for r in rows if r["dim"] == 181 and r["mode"] == "mixed"), None)
-        line = {"metric": "k-pts/s by basis dimension (16 bands, eigenvalues only); value: Si n=12 (dim 181) precision form",
+        what = "eigenvalues + eigenvectors" if args.dims_vectors else "eigenvalues only"
+        line = {"metric": f"k-pts/s by basis dimension (16 bands, {what}); value: Si n=12 (dim 181) precision form",
                 "value": head["kpts_per_s"] if head else None, "unit": "k-pts/s", "n_gpus": 1, "higher_is_better": True,
                 "data": "synthetic", "dtype": "c64+f64",
                 "config": {"workload": "Si n=6..14 and Ge+SOC n=6..14, non-local, nb 16, seeded BZ k-points",
