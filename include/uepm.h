@@ -210,7 +210,7 @@ typedef struct uepm_config {
     int32_t bt_bands;  /* back-transform: 0 automatic (above dim 192, and UEPM_MIXED
                           plans with nb + 4 > 16: the blocked MFMA kernel while X
                           fits the LDS -- complex64 while nb + 4 <= 32, FP64 in
-                          passes of 16 bands up to dim 493 at 160 KB; otherwise
+                          passes of 16 bands up to dim 524 at 160 KB; otherwise
                           16); 4, 8 or 16 bands per workgroup of the
                           reflector-by-reflector kernel                               */
     int32_t max_chunk; /* k-points per launch (0 = automatic: <= 4096, <= 4 GiB of H) */

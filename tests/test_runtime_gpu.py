@@ -140,12 +140,11 @@ def test_wy_back_transform_matches_per_reflector(gpu_ctx, sym, n, soc, nb):
                                           ("Ge", 14, True, 16)])
 def test_fp64_wy_back_transform_matches_per_reflector(gpu_ctx, sym, n, soc, nb):
     """FP64 form above dim 192: the compact-WY back-transform on FP64 MFMA
-    (automatic, bt_bands 0; dims 274, 411 in two passes of 16 bands, 458) and the
-    reflector-by-reflector kernel (bt_bands 16) apply the same Q to the same
-    tridiagonal eigenvectors, so the eigenvectors agree elementwise to rounding
-    and the Hellmann-Feynman gradients with them; the eigenvalues come before
-    the back-transform and are identical.  At dim 518 X does not fit the LDS and
-    the automatic choice is the reflector kernel itself."""
+    (automatic, bt_bands 0; dims 274, 411 in two passes of 16 bands, 458, and 518
+    with X at the unpadded LDS stride) and the reflector-by-reflector kernel
+    (bt_bands 16) apply the same Q to the same tridiagonal eigenvectors, so the
+    eigenvectors agree elementwise to rounding and the Hellmann-Feynman gradients
+    with them; the eigenvalues come before the back-transform and are identical."""
     print(f"case: FP64 WY vs per-reflector back-transform, {sym} n={n} soc={soc} nb={nb}", flush=True)
     flags = api.VECTORS | api.GRADIENTS | (api.SOC if soc else 0)
     p = api.Plan(gpu_ctx, load(sym), api.basis_vectors(n), flags, nb)

@@ -465,22 +465,25 @@ __global__ __launch_bounds__(256, 2) void apply_q_wy_kernel(const float2* __rest
 // puts output item v of lane l at element (4 v + l / 16, l % 16) (the FP32
 // one at (4 (l / 16) + v, l % 16)).  The four waves' partial [G | W] are added
 // in two levels, (w0 + w2) + (w1 + w3), a fixed order in half the LDS of a
-// four-way table.  As the FP64 form's last stage it also counts the k-points
+// four-way table.  X's row stride XS is 17 complex elements (conflict-free
+// column walks of the load, store and gradient loops) where that fits the LDS,
+// else 16 (those one-off loops then conflict; the per-block MFMA loops walk
+// rows and do not).  As the FP64 form's last stage it also counts the k-points
 // that kstat flags into status (first pass only) and writes the gradients
 // (the reflector kernel's sums, in its order).
+template <int XS>
 __global__ __launch_bounds__(256) void apply_q_wy_f64_kernel(
     const double2* __restrict__ V, int n, int ld, const double2* __restrict__ tauall, const double* __restrict__ Z,
     int nb, int b0, int nbp, double2* __restrict__ X, double* __restrict__ grads, DevParams P,
     const double* __restrict__ kpts, const int32_t* __restrict__ G, const int* __restrict__ kstat,
     unsigned long long* __restrict__ status, int64_t k0) {
     typedef double d4 __attribute__((ext_vector_type(4)));
-    constexpr int XS = 17;  // padded row stride of X in LDS (complex elements)
     extern __shared__ __align__(16) unsigned char wy64_smem[];
     double2* Xs = reinterpret_cast<double2*>(wy64_smem);  // X[row][band] at row * XS + band
     __shared__ d4 PP[2][4][64];  // partials (G re, G im, W re, W im): waves 2, 3, then w0 + w2, w1 + w3
     __shared__ double2 Gs[16][17];  // G = V^H V
-    __shared__ double2 Ws[16][XS];  // W = V^H X
-    __shared__ double2 Wq[16][XS];  // W' = T W
+    __shared__ double2 Ws[16][17];  // W = V^H X
+    __shared__ double2 Wq[16][17];  // W' = T W
     __shared__ double2 tb[16];      // tau of the block's reflectors
 
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -698,14 +701,19 @@ hipError_t launch_cluster_orth(const double* dd, const double* de, const double*
 }
 
 namespace {
-// dynamic LDS of apply_q_wy_f64_kernel (X in LDS) and its static part
-size_t wy64_lds_bytes(int n) { return sizeof(double2) * (size_t)n * 17; }
+// dynamic LDS of apply_q_wy_f64_kernel<XS> (X in LDS) and its static part
+size_t wy64_lds_bytes(int n, int xs) { return sizeof(double2) * (size_t)n * xs; }
 constexpr size_t kWy64StaticBytes = sizeof(double) * 4 * 2 * 4 * 64 + sizeof(double2) * (3 * 16 * 17 + 16);
+// row stride of X: 17 where it fits the LDS, else 16, else 0 (does not fit)
+int wy64_stride(int n, int lds_limit) {
+    if (n < 2) return 0;
+    for (int xs : {17, 16})
+        if (wy64_lds_bytes(n, xs) + kWy64StaticBytes <= (size_t)lds_limit) return xs;
+    return 0;
+}
 }  // namespace
 
-bool apply_q_wy_f64_fits(int n, int lds_limit) {
-    return n >= 2 && wy64_lds_bytes(n) + kWy64StaticBytes <= (size_t)lds_limit;
-}
+bool apply_q_wy_f64_fits(int n, int lds_limit) { return wy64_stride(n, lds_limit) > 0; }
 
 hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* dtau, const double* dZ, int nb,
                               double2* dX, double* dgrads, const DevParams& P, const double* dk, const int32_t* dG,
@@ -713,14 +721,15 @@ hipError_t launch_apply_q_f64(const double2* dV, int n, int ld, const double2* d
                               int bands_per_group, int lds_limit, hipStream_t s) {
     if (bands_per_group == 0) {
         // the blocked MFMA back-transform, sixteen bands per pass
-        if (!apply_q_wy_f64_fits(n, lds_limit)) return hipErrorInvalidValue;
-        const size_t shm = wy64_lds_bytes(n);
-        hipError_t err = hipFuncSetAttribute((const void*)apply_q_wy_f64_kernel,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        const int xs = wy64_stride(n, lds_limit);
+        if (xs == 0) return hipErrorInvalidValue;
+        const size_t shm = wy64_lds_bytes(n, xs);
+        auto kern = xs == 17 ? apply_q_wy_f64_kernel<17> : apply_q_wy_f64_kernel<16>;
+        hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (err != hipSuccess) return err;
         for (int b0 = 0; b0 < nb; b0 += 16)
-            hipLaunchKernelGGL(apply_q_wy_f64_kernel, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb,
-                               b0, std::min(16, nb - b0), dX, dgrads, P, dk, dG, kstat, status, k0);
+            hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(256), shm, s, dV, n, ld, dtau, dZ, nb, b0,
+                               std::min(16, nb - b0), dX, dgrads, P, dk, dG, kstat, status, k0);
         return hipGetLastError();
     }
     return launch_apply<double, double2>(dV, n, ld, dtau, dZ, nb, dX, dgrads, P, dk, dG, nk, kstat, status, k0,
