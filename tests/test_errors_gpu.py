@@ -44,15 +44,19 @@ def test_host_solve_refuses_non_finite_k(gpu_ctx):
     p.close()
 
 
-@pytest.mark.parametrize("flags,nb", [(api.NONLOCAL, 8), (api.NONLOCAL | api.GRADIENTS, 8),
-                                      (api.NONLOCAL | api.MIXED, 12), (api.NONLOCAL | api.MIXED | api.VECTORS, 12),
-                                      (api.NONLOCAL | api.SOC | api.MIXED, 16)])
-def test_device_batch_with_non_finite_k(gpu_ctx, oracle, flags, nb):
-    """NaN and infinite k-points inside a device batch spanning two chunks (two lanes)."""
+@pytest.mark.parametrize("flags,nb,shells", [(api.NONLOCAL, 8, 8), (api.NONLOCAL | api.GRADIENTS, 8, 8),
+                                             (api.NONLOCAL | api.MIXED, 12, 8),
+                                             (api.NONLOCAL | api.MIXED | api.VECTORS, 12, 8),
+                                             (api.NONLOCAL | api.SOC | api.MIXED, 16, 8),
+                                             (api.NONLOCAL | api.SOC | api.GRADIENTS, 16, 10)])
+def test_device_batch_with_non_finite_k(gpu_ctx, oracle, flags, nb, shells):
+    """NaN and infinite k-points inside a device batch spanning two chunks (two
+    lanes); at Ge + SOC n = 10 (dim 274) the FP64 form's last stage is the
+    compact-WY back-transform, which reports the failed k-points."""
     sym = "Ge" if flags & api.SOC else "Si"
-    print(f"case: device batch with NaN/inf k, {sym} n=8 flags {flags} nb {nb}", flush=True)
+    print(f"case: device batch with NaN/inf k, {sym} n={shells} flags {flags} nb {nb}", flush=True)
     mat = load(sym)
-    G = api.basis_vectors(8)
+    G = api.basis_vectors(shells)
     ks = api.bz_sample(31, 90)
     bad = [5, 61, 62]
     ks[5, 0] = np.nan

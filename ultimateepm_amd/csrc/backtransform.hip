@@ -615,10 +615,14 @@ __global__ __launch_bounds__(256) void apply_q_wy_f64_kernel(
         }
         __syncthreads();
     }
-    // eigenvectors X[k][band][row]
+    // eigenvectors X[k][band][row]; a k-point the eigen-stage flagged gets NaN
+    // throughout (its z is NaN, but row 0, which no reflector touches, would
+    // keep a zero imaginary part)
+    const bool failed = kstat && kstat[kk];
     for (int e = tid; e < n * nbp; e += 256) {
         const int b = e / n, row = e - b * n;
-        X[(kk * nb + b0 + b) * (int64_t)n + row] = Xs[row * XS + b];
+        const double nan = __builtin_nan("");
+        X[(kk * nb + b0 + b) * (int64_t)n + row] = failed ? make_double2(nan, nan) : Xs[row * XS + b];
     }
     if (grads) {
         // Hellmann-Feynman on the kinetic term, 16 lanes per band (rows rs + 16 q
