@@ -231,6 +231,54 @@ def test_batch_graph_matches_individual_solves(gpu_ctx):
         p.close()
 
 
+def test_batch_graph_with_vectors_matches_individual_solves(gpu_ctx):
+    """Batch items with eigenvectors (and gradients): the captured graph holds the
+    FP64 compact-WY back-transform (Ge + SOC dim 274), the complex64 one with the
+    precision form's refinement (dim 274) and the per-reflector kernel (Si dim 89);
+    every replay equals the plans' own solves (to 1e-12)."""
+    print("case: batch graph with eigenvectors, 3 plans, 2 replays", flush=True)
+    specs = [("Ge", 10, api.NONLOCAL | api.SOC | api.GRADIENTS, 16), ("Ge", 10, api.NONLOCAL | api.SOC | api.MIXED |
+                                                                      api.VECTORS, 16),
+             ("Si", 8, api.NONLOCAL | api.GRADIENTS, 8)]
+    ctx = gpu_ctx
+    plans, ks, dks, outs, ref = [], [], [], [], []
+    for i, (sym, n, flags, nb) in enumerate(specs):
+        p = api.Plan(ctx, load(sym), api.basis_vectors(n), flags, nb)
+        k = api.bz_sample(300 + i, 150 + 40 * i)
+        ev, vec, gr = p.solve(k, vectors=True)  # a UEPM_GRADIENTS plan computes the vectors too
+        ref.append((ev, vec, gr))
+        plans.append(p)
+        ks.append(k)
+        dks.append(_dev(ctx, k))
+        de = api.DeviceArray(ctx, (len(k), nb))
+        dv = api.DeviceArray(ctx, (len(k), nb, p.dim, 2))
+        dg = api.DeviceArray(ctx, (len(k), nb, 3)) if flags & api.GRADIENTS else None
+        outs.append((de, dv, dg))
+    items = [(p, dk.ptr, len(k), de.ptr, dv.ptr, dg.ptr if dg else 0)
+             for p, dk, k, (de, dv, dg) in zip(plans, dks, ks, outs)]
+    b = api.Batch(ctx, items, branches=3)
+    for replay in range(2):
+        for de, dv, dg in outs:
+            de.zero()
+            dv.zero()
+            if dg:
+                dg.zero()
+        b.launch()
+        ctx.synchronize()
+        for p, k in zip(plans, ks):  # eager work between the replays
+            p.solve(k[:5])
+        for i, (de, dv, dg) in enumerate(outs):
+            ev, vec, gr = ref[i]
+            assert np.abs(de.numpy() - ev).max() <= 1e-12, (replay, specs[i])
+            v = dv.numpy()
+            assert np.abs(v[..., 0] + 1j * v[..., 1] - vec).max() <= 1e-12, (replay, specs[i])
+            if dg:
+                assert np.abs(dg.numpy() - gr).max() <= 1e-12 * np.abs(gr).max(), (replay, specs[i])
+    b.close()
+    for p in plans:
+        p.close()
+
+
 def _workload(seed, count):
     return serving.poisson_workload(seed, count, rate=400.0, n_min=6, n_max=10, k_max=256)
 
