@@ -19,10 +19,10 @@ struct TriPlan {
     int64_t scratch_elems;  // complex elements of global scratch per workgroup
 };
 TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs);
-// s_end < n - 1 (a multiple of tridiag_blocked_panel()) stops after step
-// s_end - 1 and leaves the updated trailing block A[s_end:, s_end:] (lower
-// triangle) at its place in Vout, for launch_tridiag_reg_c128 to continue.
-int tridiag_blocked_panel();
+// s_end < n - 1 (a multiple of tridiag_blocked_panel(element bytes)) stops after
+// step s_end - 1 and leaves the updated trailing block A[s_end:, s_end:] (lower
+// triangle) at its place in Vout, for the register grids to continue.
+int tridiag_blocked_panel(int elem_bytes);
 hipError_t launch_tridiag_blocked_f64(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                       double2* Vout, double* dd, double* de, double2* dtau, hipStream_t s,
                                       int s_end = 1 << 30);

@@ -93,7 +93,10 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // Column padding: length congruent to 2 modulo MOD (elements per bank row).
 __host__ __device__ inline int padded_len(int L, int MOD) { return L + (((2 - L) % MOD) + MOD) % MOD; }
 
-constexpr int kTriNBP = 4;
+// panel width: 8 in complex64, 4 in complex128 (whose panel columns V, W at
+// width 8 would not leave the LDS room at the largest dims)
+template <typename T>
+constexpr int tri_nbp() { return sizeof(T) == 4 ? 8 : 4; }
 constexpr int kTriParts = 4;
 constexpr int kTriMaxRowsPerLane = 9;  // rows per lane in the symmetric product: n <= 576
 static_assert(64 * kTriMaxRowsPerLane >= UEPM_MAX_DIM, "every plan dimension fits the symmetric product");
@@ -632,8 +635,8 @@ TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs) {
     tp.n = n;
     tp.bs = bs;
     const int MOD = 128 / elem_bytes;
-    const int fixed = elem_bytes == 16 ? TriSmem<double>::fixed_bytes(n, bs, kTriNBP)
-                                       : TriSmem<float>::fixed_bytes(n, bs, kTriNBP);
+    const int fixed = elem_bytes == 16 ? TriSmem<double>::fixed_bytes(n, bs, tri_nbp<double>())
+                                       : TriSmem<float>::fixed_bytes(n, bs, tri_nbp<float>());
     const int avail = (lds_limit - fixed) / elem_bytes;
     int c0 = n, used = 0;
     for (int c = n - 1; c >= 0; --c) {
@@ -650,12 +653,12 @@ TriPlan tri_plan(int n, int elem_bytes, int lds_limit, int bs) {
     return tp;
 }
 
-int tridiag_blocked_panel() { return kTriNBP; }
+int tridiag_blocked_panel(int elem_bytes) { return elem_bytes == 16 ? tri_nbp<double>() : tri_nbp<float>(); }
 
 template <typename T, int BS>
 static hipError_t launch_tri_blocked_bs(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                         void* Vout, double* dd, double* de, void* dtau, int s_end, hipStream_t s) {
-    auto kern = tridiag_blocked_kernel<T, BS, kTriNBP>;
+    auto kern = tridiag_blocked_kernel<T, BS, tri_nbp<T>()>;
     hipError_t err = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, tp.lds_bytes);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(kern, dim3((unsigned)nk), dim3(BS), tp.lds_bytes, s, dH, tp.n, ld, tp.c0,
@@ -667,7 +670,7 @@ static hipError_t launch_tri_blocked_bs(const TriPlan& tp, const double2* dH, in
 template <typename T>
 static hipError_t launch_tri_blocked_t(const TriPlan& tp, const double2* dH, int ld, int64_t nk, void* scratch,
                                        void* Vout, double* dd, double* de, void* dtau, int s_end, hipStream_t s) {
-    if (s_end < tp.n - 1 && (s_end < 0 || s_end % kTriNBP != 0)) return hipErrorInvalidValue;
+    if (s_end < tp.n - 1 && (s_end < 0 || s_end % tri_nbp<T>() != 0)) return hipErrorInvalidValue;
     if (tp.n > 64 * kTriMaxRowsPerLane) return hipErrorInvalidValue;  // rows per lane in the product
     if (tp.bs >= 512) return launch_tri_blocked_bs<T, 512>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);
     return launch_tri_blocked_bs<T, 256>(tp, dH, ld, nk, scratch, Vout, dd, de, dtau, s_end, s);

@@ -730,7 +730,7 @@ hipError_t launch_tridiag_reg_c64(const double2* dH, int n, int ld, int64_t nk, 
 int tridiag_reg_c64_tail(int n) {
     if (n < 2) return 0;
     if (n <= kGrid[0]) return n;
-    const int P = tridiag_blocked_panel();
+    const int P = tridiag_blocked_panel((int)sizeof(float2));
     const int s0 = ((n - kGrid[0] + P - 1) / P) * P;
     return n - s0;
 }
@@ -775,8 +775,8 @@ void grid64_stage(int g, const double2* H, int64_t hk, int m, int ld, int64_t nk
 int tridiag_reg_c128_tail(int n) {
     if (n < 2) return 0;
     if (n <= kGrid64[0]) return n;
-    // the blocked reduction stops at a panel boundary s0 >= n - 128
-    const int P = tridiag_blocked_panel();
+    // the blocked reduction stops at a panel boundary s0 >= n - 192
+    const int P = tridiag_blocked_panel((int)sizeof(double2));
     const int s0 = ((n - kGrid64[0] + P - 1) / P) * P;
     return n - s0;
 }
